@@ -1,0 +1,213 @@
+/*
+ * zynum_gpu.h -- C ABI of the MI355X (gfx950) Level-3 tier.
+ *
+ * Part 1 keeps Zynum's public BLAS symbols byte-for-byte (32-bit LP64 BLAS
+ * integers, Fortran arguments by pointer with no hidden string lengths).  A
+ * call whose operands live in device memory runs on the GPU; host operands are
+ * staged through device memory by the same kernels.  Part 2 is the private
+ * request ABI the host planner (or any caller holding a HIP stream) uses to
+ * reach the GPU tier directly, including the split-precision and grouped
+ * forms.  All signatures use plain pointers and sizes only.
+ *
+ * Reference interfaces (paths relative to the Zynum source tree):
+ *   cblas_sgemm / cblas_dgemm  include/zynum/blas/cblas.h:1336,1353,
+ *                              src/blas/abi/cblas.zig:1415-1439
+ *   sgemm_ / dgemm_            include/zynum/blas/blas.h:161-162,
+ *                              src/blas/abi/fortran.zig:760-767
+ *   dsyrk_ / cblas_dsyrk       src/blas/abi/fortran.zig:806-809,
+ *                              src/blas/abi/cblas.zig:1550-1559, cblas.h:1514
+ *   dtrsm_ / cblas_dtrsm       src/blas/abi/fortran.zig:873-876,
+ *                              src/blas/abi/cblas.zig:1742-1753, cblas.h:1755
+ *   ssyrk_ / strsm_            src/blas/abi/fortran.zig:802-805,869-872
+ *   xerbla_ / lsame_           src/blas/abi/fortran.zig:16-22
+ *   zynum_blas_shutdown(_)     src/blas/abi/fortran.zig:41-47
+ *   zynum_gpu_gemm_request     modeled on GemmRequest,
+ *                              src/blas/kernels/isolated/x86_64_structured_abi.zig:50-70
+ *                              (first 88 bytes identical; GPU fields appended)
+ */
+#ifndef ZYNUM_GPU_H
+#define ZYNUM_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* every symbol declared here is exported from libzynum_gpu.so */
+#pragma GCC visibility push(default)
+
+typedef int32_t zynum_blas_int;
+
+/* ---------------------------------------------------------------------- */
+/* Part 1: Zynum public BLAS symbols                                       */
+/* ---------------------------------------------------------------------- */
+
+enum {
+    CblasRowMajor = 101, CblasColMajor = 102,
+    CblasNoTrans = 111, CblasTrans = 112, CblasConjTrans = 113,
+    CblasUpper = 121, CblasLower = 122,
+    CblasNonUnit = 131, CblasUnit = 132,
+    CblasLeft = 141, CblasRight = 142
+};
+
+void cblas_sgemm(int layout, int transa, int transb, int m, int n, int k, float alpha,
+                 const float *a, int lda, const float *b, int ldb, float beta, float *c, int ldc);
+void cblas_dgemm(int layout, int transa, int transb, int m, int n, int k, double alpha,
+                 const double *a, int lda, const double *b, int ldb, double beta, double *c, int ldc);
+void cblas_dsyrk(int layout, int uplo, int trans, int n, int k, double alpha, const double *a,
+                 int lda, double beta, double *c, int ldc);
+void cblas_dtrsm(int layout, int side, int uplo, int transa, int diag, int m, int n, double alpha,
+                 const double *a, int lda, double *b, int ldb);
+
+void sgemm_(const char *transa, const char *transb, const zynum_blas_int *m, const zynum_blas_int *n,
+            const zynum_blas_int *k, const float *alpha, const float *a, const zynum_blas_int *lda,
+            const float *b, const zynum_blas_int *ldb, const float *beta, float *c,
+            const zynum_blas_int *ldc);
+void dgemm_(const char *transa, const char *transb, const zynum_blas_int *m, const zynum_blas_int *n,
+            const zynum_blas_int *k, const double *alpha, const double *a, const zynum_blas_int *lda,
+            const double *b, const zynum_blas_int *ldb, const double *beta, double *c,
+            const zynum_blas_int *ldc);
+void ssyrk_(const char *uplo, const char *trans, const zynum_blas_int *n, const zynum_blas_int *k,
+            const float *alpha, const float *a, const zynum_blas_int *lda, const float *beta,
+            float *c, const zynum_blas_int *ldc);
+void dsyrk_(const char *uplo, const char *trans, const zynum_blas_int *n, const zynum_blas_int *k,
+            const double *alpha, const double *a, const zynum_blas_int *lda, const double *beta,
+            double *c, const zynum_blas_int *ldc);
+void strsm_(const char *side, const char *uplo, const char *transa, const char *diag,
+            const zynum_blas_int *m, const zynum_blas_int *n, const float *alpha, const float *a,
+            const zynum_blas_int *lda, float *b, const zynum_blas_int *ldb);
+void dtrsm_(const char *side, const char *uplo, const char *transa, const char *diag,
+            const zynum_blas_int *m, const zynum_blas_int *n, const double *alpha, const double *a,
+            const zynum_blas_int *lda, double *b, const zynum_blas_int *ldb);
+
+int lsame_(const char *ca, const char *cb);
+void xerbla_(const char *srname, const zynum_blas_int *info);
+void zynum_blas_shutdown(void);
+void zynum_blas_shutdown_(void);
+
+/* ---------------------------------------------------------------------- */
+/* Part 2: GPU-tier private ABI (no counterpart in the reference beyond    */
+/* the GemmRequest layout it extends)                                      */
+/* ---------------------------------------------------------------------- */
+
+/* Return codes.  Every nonzero code except ZYNUM_GPU_ERR_LAUNCH is returned
+   before any caller-visible memory is written (decline-before-mutation). */
+enum {
+    ZYNUM_GPU_OK = 0,
+    ZYNUM_GPU_DECLINE_NO_DEVICE = 1,   /* no usable gfx950 device */
+    ZYNUM_GPU_DECLINE_ALLOC = 2,       /* staging or workspace allocation failed */
+    ZYNUM_GPU_DECLINE_UNSUPPORTED = 3, /* form/mode/scalar not provided */
+    ZYNUM_GPU_DECLINE_ARGUMENT = 4,    /* invalid argument (xerbla-style check failed) */
+    ZYNUM_GPU_DECLINE_FORCED = 5,      /* decline injected by zynum_gpu_debug_force_decline */
+    ZYNUM_GPU_ERR_LAUNCH = 6           /* HIP error after launch; outputs undefined */
+};
+
+enum { ZYNUM_GPU_F32 = 0, ZYNUM_GPU_F64 = 1 };
+enum { ZYNUM_GPU_NO_TRANS = 0, ZYNUM_GPU_TRANS = 1, ZYNUM_GPU_CONJ_TRANS = 2 };
+
+/* Precision modes of a DGEMM request. */
+enum {
+    ZYNUM_GPU_MODE_PLAIN = 0,      /* FP64 (or FP32) MFMA */
+    ZYNUM_GPU_MODE_OZAKI_INT8 = 1, /* split precision: mode_param INT8 slices (2..8), INT32 accumulation */
+    ZYNUM_GPU_MODE_OZAKI_BF16 = 2, /* split precision: mode_param BF16 slices (2..8), FP32 accumulation */
+    ZYNUM_GPU_MODE_OZAKI_CRT = 3,  /* split precision: mode_param INT8 residue GEMMs (8..20) reconstructed
+                                      by the Chinese remainder theorem (Ozaki scheme II) */
+    ZYNUM_GPU_MODE_DD = 4          /* double-double FP64 VALU reference (accuracy oracle on the device) */
+};
+
+/* request flags */
+enum {
+    ZYNUM_GPU_FLAG_ASYNC = 1 /* do not synchronize the stream before returning */
+};
+
+typedef struct zynum_gpu_gemm_request {
+    uint8_t transa;      /* ZYNUM_GPU_NO_TRANS / TRANS / CONJ_TRANS */
+    uint8_t transb;
+    uint8_t scalar;      /* ZYNUM_GPU_F32 / ZYNUM_GPU_F64 */
+    uint8_t reserved0[5];
+    int32_t m, n, k, lda, ldb, ldc;
+    uint64_t alpha_re;   /* bit pattern of alpha (double, or float in the low 32 bits) */
+    uint64_t alpha_im;   /* must be 0 */
+    uint64_t beta_re;
+    uint64_t beta_im;    /* must be 0 */
+    const void *a;
+    const void *b;
+    void *c;
+    /* GPU-tier fields (offset 88) */
+    void *stream;        /* hipStream_t; NULL uses the stream set by zynum_gpu_set_stream */
+    uint8_t mode;        /* ZYNUM_GPU_MODE_* */
+    uint8_t mode_param;  /* slices or moduli count; 0 picks the default */
+    uint8_t flags;       /* ZYNUM_GPU_FLAG_* */
+    uint8_t reserved1[5];
+} zynum_gpu_gemm_request; /* 104 bytes */
+
+/* Run one GEMM request: C = alpha*op(A)*op(B) + beta*C (column-major).
+   Arguments follow dgemm_ rules (gemmError, fortran.zig:88-103). */
+int zynum_gpu_gemm(const zynum_gpu_gemm_request *request);
+
+/* One problem of a grouped GEMM batch (device pointers, column-major). */
+typedef struct zynum_gpu_gemm_problem {
+    uint8_t transa, transb, reserved0[6];
+    int32_t m, n, k, lda, ldb, ldc;
+    double alpha, beta;  /* used as float for an FP32 batch */
+    const void *a;
+    const void *b;
+    void *c;
+} zynum_gpu_gemm_problem; /* 72 bytes */
+
+/* Opaque grouped-GEMM plan: the problem list is validated, bucketed by size,
+   assigned a tile shape per bucket by the autotuner table, and uploaded once.
+   zynum_gpu_grouped_run issues one persistent launch per batch; it allocates
+   nothing and may be captured into a HIP graph. */
+typedef struct zynum_gpu_grouped_plan zynum_gpu_grouped_plan;
+
+int zynum_gpu_grouped_plan_create(int scalar, const zynum_gpu_gemm_problem *problems, int count,
+                                  zynum_gpu_grouped_plan **plan_out);
+int zynum_gpu_grouped_run(zynum_gpu_grouped_plan *plan, void *stream);
+/* Capture the persistent launch into a HIP graph owned by the plan and replay it. */
+int zynum_gpu_grouped_graph_run(zynum_gpu_grouped_plan *plan, void *stream);
+int zynum_gpu_grouped_plan_info(const zynum_gpu_grouped_plan *plan, int64_t *tiles_out,
+                                int64_t *workgroups_out, double *flops_out);
+void zynum_gpu_grouped_plan_destroy(zynum_gpu_grouped_plan *plan);
+
+/* Reference loop for the same batch: one plain GEMM launch per problem. */
+int zynum_gpu_grouped_loop(int scalar, const zynum_gpu_gemm_problem *problems, int count, void *stream);
+
+/* Level-3 on the GEMM core, device pointers, stream-ordered.  Arguments follow
+   dtrsm_/dsyrk_ (triangularMatrixError, rankKMatrixError).  `mode` is
+   ZYNUM_GPU_MODE_PLAIN or a split-precision mode used for the off-diagonal
+   GEMM updates (diagonal blocks are always solved in FP64). */
+int zynum_gpu_dtrsm(char side, char uplo, char transa, char diag, int m, int n, double alpha,
+                    const double *a, int lda, double *b, int ldb, void *stream, int mode,
+                    int mode_param, int flags);
+int zynum_gpu_dsyrk(char uplo, char trans, int n, int k, double alpha, const double *a, int lda,
+                    double beta, double *c, int ldc, void *stream, int mode, int mode_param,
+                    int flags);
+
+/* Stream used by the Part 1 symbols (default: the null stream). */
+void zynum_gpu_set_stream(void *stream);
+/* Last return code seen by a Part 1 symbol on this thread. */
+int zynum_gpu_last_status(void);
+/* Test hook: the next `count` GPU-tier calls decline with ZYNUM_GPU_DECLINE_FORCED. */
+void zynum_gpu_debug_force_decline(int count);
+/* Human-readable text for a return code. */
+const char *zynum_gpu_status_string(int status);
+/* Library version string. */
+const char *zynum_gpu_version(void);
+
+/* Double-double reference product on the device (FP64 VALU, TwoProd/TwoSum):
+   hi + lo ~= alpha*op(A)*op(B) + beta*C0 to about 2^-104 relative.  C0 may be
+   NULL when beta == 0; lo may be NULL.  Synchronous.  Used to state the
+   error of the split-precision forms. */
+int zynum_gpu_dd_gemm(char transa, char transb, int m, int n, int k, double alpha, const double *a,
+                      int lda, const double *b, int ldb, double beta, const double *c0, int ldc0,
+                      double *hi, double *lo, int ldo, void *stream);
+
+#pragma GCC visibility pop
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ZYNUM_GPU_H */

@@ -1,0 +1,392 @@
+// Exported BLAS symbols and the private GPU-tier request ABI.
+//
+// Argument checking, xerbla reporting and CBLAS row-major normalisation keep
+// Zynum's semantics (src/blas/abi/fortran.zig:16-47,88-117,252-262,277-282,
+// 760-876; src/blas/abi/cblas.zig:16-116,1415-1439,1550-1559,1742-1753).
+// Execution is always on the GPU: device operands are used in place and host
+// operands are staged through device memory.  There is no CPU compute path.
+#include <ctype.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+
+#include "gemm.hpp"
+#include "ops.hpp"
+#include "runtime.hpp"
+
+namespace {
+
+thread_local int t_last_status = ZYNUM_GPU_OK;
+
+inline char up(const char* p) { return (char)toupper((unsigned char)p[0]); }
+inline bool valid_trans(char c) { return c == 'N' || c == 'T' || c == 'C'; }
+inline bool valid_uplo(char c) { return c == 'U' || c == 'L'; }
+inline bool valid_diag(char c) { return c == 'N' || c == 'U'; }
+inline bool valid_side(char c) { return c == 'L' || c == 'R'; }
+inline int32_t max1(int32_t x) { return x > 1 ? x : 1; }
+
+// fortran.zig:88-103
+int gemm_error(char ta, char tb, int32_t m, int32_t n, int32_t k, int32_t lda, int32_t ldb, int32_t ldc) {
+    if (!valid_trans(ta)) return 1;
+    if (!valid_trans(tb)) return 2;
+    if (m < 0) return 3;
+    if (n < 0) return 4;
+    if (k < 0) return 5;
+    const int32_t nrowa = (ta == 'N') ? m : k;
+    const int32_t nrowb = (tb == 'N') ? k : n;
+    if (lda < max1(nrowa)) return 8;
+    if (ldb < max1(nrowb)) return 10;
+    if (ldc < max1(m)) return 13;
+    return 0;
+}
+
+// fortran.zig:105-117
+int triangular_error(char side, char uplo, char trans, char diag, int32_t m, int32_t n, int32_t lda, int32_t ldb) {
+    if (!valid_side(side)) return 1;
+    if (!valid_uplo(uplo)) return 2;
+    if (!valid_trans(trans)) return 3;
+    if (!valid_diag(diag)) return 4;
+    if (m < 0) return 5;
+    if (n < 0) return 6;
+    const int32_t nrowa = (side == 'L') ? m : n;
+    if (lda < max1(nrowa)) return 9;
+    if (ldb < max1(m)) return 11;
+    return 0;
+}
+
+// fortran.zig:252-262
+int rank_k_error(char uplo, char trans, int32_t n, int32_t k, int32_t lda, int32_t ldc) {
+    if (!valid_uplo(uplo)) return 1;
+    if (!valid_trans(trans)) return 2;
+    if (n < 0) return 3;
+    if (k < 0) return 4;
+    const int32_t nrowa = (trans == 'N') ? n : k;
+    if (lda < max1(nrowa)) return 7;
+    if (ldc < max1(n)) return 10;
+    return 0;
+}
+
+// fortran.zig:277-282: report through xerbla_ with the padded routine name
+bool report_error(const char* name, int info) {
+    if (info == 0) return false;
+    zynum_blas_int i = info;
+    xerbla_(name, &i);
+    return true;
+}
+
+void note_status(const char* routine, int st) {
+    t_last_status = st;
+    if (st != ZYNUM_GPU_OK && st != ZYNUM_GPU_DECLINE_FORCED) {
+        fprintf(stderr, " ** %s: GPU tier did not run (%s); outputs %s\n", routine, zynum_gpu_status_string(st),
+                st == ZYNUM_GPU_ERR_LAUNCH ? "undefined" : "unchanged");
+    }
+}
+
+template <typename T>
+T scalar_from_bits(uint64_t bits);
+template <>
+double scalar_from_bits<double>(uint64_t bits) {
+    double v;
+    memcpy(&v, &bits, sizeof(v));
+    return v;
+}
+template <>
+float scalar_from_bits<float>(uint64_t bits) {
+    uint32_t lo = (uint32_t)bits;
+    float v;
+    memcpy(&v, &lo, sizeof(v));
+    return v;
+}
+
+}  // namespace
+
+namespace zg {
+
+template <typename T>
+int gemm_op(bool ta, bool tb, int m, int n, int k, T alpha, const T* a, int64_t lda, const T* b, int64_t ldb,
+            T beta, T* c, int64_t ldc, hipStream_t s, int mode, int mode_param, int flags) {
+    if (m <= 0 || n <= 0) return ZYNUM_GPU_OK;  // gemm.zig:1393
+    if (take_forced_decline()) return ZYNUM_GPU_DECLINE_FORCED;
+    if (!device_ready()) return ZYNUM_GPU_DECLINE_NO_DEVICE;
+    if (mode != ZYNUM_GPU_MODE_PLAIN && sizeof(T) != 8) return ZYNUM_GPU_DECLINE_UNSUPPORTED;
+    if (mode > ZYNUM_GPU_MODE_DD) return ZYNUM_GPU_DECLINE_UNSUPPORTED;
+
+    const bool reads_ab = (k > 0) && (alpha != T(0));
+    Operand oa, ob, oc;
+    int st = ZYNUM_GPU_OK;
+    if (reads_ab) {
+        st = oa.bind_in(a, ta ? k : m, ta ? m : k, lda, sizeof(T), s);
+        if (st == ZYNUM_GPU_OK) st = ob.bind_in(b, tb ? n : k, tb ? k : n, ldb, sizeof(T), s);
+    }
+    if (st == ZYNUM_GPU_OK) st = oc.bind_out(c, m, n, ldc, sizeof(T), beta != T(0), s);
+    if (st != ZYNUM_GPU_OK) return st;
+
+    const T* da = static_cast<const T*>(oa.dev);
+    const T* db = static_cast<const T*>(ob.dev);
+    T* dc = static_cast<T*>(oc.dev);
+    if (!reads_ab || mode == ZYNUM_GPU_MODE_PLAIN) {
+        GemmDesc<T> d;
+        d.ta = ta; d.tb = tb; d.m = m; d.n = n; d.k = k;
+        d.lda = lda; d.ldb = ldb; d.ldc = ldc;
+        d.alpha = alpha; d.beta = beta;
+        d.a = da; d.b = db; d.c = dc;
+        st = status_from_hip(gemm_launch<T>(d, s));
+    } else if (mode == ZYNUM_GPU_MODE_DD) {
+        if constexpr (sizeof(T) == 8) st = dd_dgemm(ta, tb, m, n, k, alpha, da, lda, db, ldb, beta, dc, ldc, s);
+    } else {
+        if constexpr (sizeof(T) == 8)
+            st = ozaki_dgemm(ta, tb, m, n, k, alpha, da, lda, db, ldb, beta, dc, ldc, s, mode, mode_param);
+    }
+    if (st != ZYNUM_GPU_OK) return st;
+    if (oc.staged) return oc.finish(s);
+    if (!(flags & ZYNUM_GPU_FLAG_ASYNC) || oa.staged || ob.staged) return status_from_hip(hipStreamSynchronize(s));
+    return ZYNUM_GPU_OK;
+}
+
+template <typename T>
+int trsm_op(char side, char uplo, bool trans, char diag, int m, int n, T alpha, const T* a, int64_t lda, T* b,
+            int64_t ldb, hipStream_t s, int mode, int mode_param, int flags) {
+    if (m <= 0 || n <= 0) return ZYNUM_GPU_OK;  // triangular.zig:234
+    if (take_forced_decline()) return ZYNUM_GPU_DECLINE_FORCED;
+    if (!device_ready()) return ZYNUM_GPU_DECLINE_NO_DEVICE;
+    if (mode != ZYNUM_GPU_MODE_PLAIN && sizeof(T) != 8) return ZYNUM_GPU_DECLINE_UNSUPPORTED;
+    const int na = (side == 'L') ? m : n;
+    Operand oa, ob;
+    int st = oa.bind_in(a, na, na, lda, sizeof(T), s);
+    if (st == ZYNUM_GPU_OK) st = ob.bind_out(b, m, n, ldb, sizeof(T), true, s);
+    if (st != ZYNUM_GPU_OK) return st;
+    st = trsm_device<T>(side, uplo, trans, diag, m, n, alpha, static_cast<const T*>(oa.dev), lda,
+                        static_cast<T*>(ob.dev), ldb, s, mode, mode_param);
+    if (st != ZYNUM_GPU_OK) return st;
+    if (ob.staged) return ob.finish(s);
+    if (!(flags & ZYNUM_GPU_FLAG_ASYNC) || oa.staged) return status_from_hip(hipStreamSynchronize(s));
+    return ZYNUM_GPU_OK;
+}
+
+template <typename T>
+int syrk_op(char uplo, bool trans, int n, int k, T alpha, const T* a, int64_t lda, T beta, T* c, int64_t ldc,
+            hipStream_t s, int mode, int mode_param, int flags) {
+    if (n <= 0) return ZYNUM_GPU_OK;  // symmetric.zig:238
+    if (take_forced_decline()) return ZYNUM_GPU_DECLINE_FORCED;
+    if (!device_ready()) return ZYNUM_GPU_DECLINE_NO_DEVICE;
+    if (mode != ZYNUM_GPU_MODE_PLAIN && sizeof(T) != 8) return ZYNUM_GPU_DECLINE_UNSUPPORTED;
+    Operand oa, oc;
+    int st = ZYNUM_GPU_OK;
+    if (k > 0) st = oa.bind_in(a, trans ? k : n, trans ? n : k, lda, sizeof(T), s);
+    // the unstored triangle must come back unchanged, so C is always copied in
+    if (st == ZYNUM_GPU_OK) st = oc.bind_out(c, n, n, ldc, sizeof(T), true, s);
+    if (st != ZYNUM_GPU_OK) return st;
+    st = syrk_device<T>(uplo, trans, n, k, alpha, static_cast<const T*>(oa.dev), lda, beta, static_cast<T*>(oc.dev),
+                        ldc, s, mode, mode_param);
+    if (st != ZYNUM_GPU_OK) return st;
+    if (oc.staged) return oc.finish(s);
+    if (!(flags & ZYNUM_GPU_FLAG_ASYNC) || oa.staged) return status_from_hip(hipStreamSynchronize(s));
+    return ZYNUM_GPU_OK;
+}
+
+template int gemm_op<double>(bool, bool, int, int, int, double, const double*, int64_t, const double*, int64_t,
+                             double, double*, int64_t, hipStream_t, int, int, int);
+template int gemm_op<float>(bool, bool, int, int, int, float, const float*, int64_t, const float*, int64_t, float,
+                            float*, int64_t, hipStream_t, int, int, int);
+
+}  // namespace zg
+
+// ---------------------------------------------------------------------------
+// Fortran ABI (fortran.zig)
+// ---------------------------------------------------------------------------
+extern "C" {
+
+int lsame_(const char* ca, const char* cb) { return toupper((unsigned char)ca[0]) == toupper((unsigned char)cb[0]); }
+
+// fortran.zig:20-39: the routine name is read up to 6 characters or a NUL and
+// trimmed of spaces.
+void xerbla_(const char* srname, const zynum_blas_int* info) {
+    char name[8];
+    size_t len = 0;
+    while (len < 6 && srname[len] != 0) {
+        name[len] = srname[len];
+        ++len;
+    }
+    size_t start = 0;
+    while (start < len && name[start] == ' ') ++start;
+    while (len > start && name[len - 1] == ' ') --len;
+    name[len] = 0;
+    const char* routine = (len > start) ? name + start : "UNKNOWN";
+    fprintf(stderr, " ** On entry to %s parameter number %d had an illegal value\n", routine, (int)*info);
+}
+
+void zynum_blas_shutdown(void) { zg::workspace_free_all(); }
+void zynum_blas_shutdown_(void) { zg::workspace_free_all(); }
+
+void sgemm_(const char* ta, const char* tb, const zynum_blas_int* m, const zynum_blas_int* n, const zynum_blas_int* k,
+            const float* alpha, const float* a, const zynum_blas_int* lda, const float* b, const zynum_blas_int* ldb,
+            const float* beta, float* c, const zynum_blas_int* ldc) {
+    const char A = up(ta), B = up(tb);
+    if (report_error("SGEMM ", gemm_error(A, B, *m, *n, *k, *lda, *ldb, *ldc))) return;
+    note_status("SGEMM", zg::gemm_op<float>(A != 'N', B != 'N', *m, *n, *k, *alpha, a, *lda, b, *ldb, *beta, c, *ldc,
+                                            zg::blas_stream(), ZYNUM_GPU_MODE_PLAIN, 0, 0));
+}
+
+void dgemm_(const char* ta, const char* tb, const zynum_blas_int* m, const zynum_blas_int* n, const zynum_blas_int* k,
+            const double* alpha, const double* a, const zynum_blas_int* lda, const double* b,
+            const zynum_blas_int* ldb, const double* beta, double* c, const zynum_blas_int* ldc) {
+    const char A = up(ta), B = up(tb);
+    if (report_error("DGEMM ", gemm_error(A, B, *m, *n, *k, *lda, *ldb, *ldc))) return;
+    note_status("DGEMM", zg::gemm_op<double>(A != 'N', B != 'N', *m, *n, *k, *alpha, a, *lda, b, *ldb, *beta, c,
+                                             *ldc, zg::blas_stream(), ZYNUM_GPU_MODE_PLAIN, 0, 0));
+}
+
+void ssyrk_(const char* u, const char* t, const zynum_blas_int* n, const zynum_blas_int* k, const float* alpha,
+            const float* a, const zynum_blas_int* lda, const float* beta, float* c, const zynum_blas_int* ldc) {
+    const char U = up(u), T = up(t);
+    if (report_error("SSYRK ", rank_k_error(U, T, *n, *k, *lda, *ldc))) return;
+    note_status("SSYRK", zg::syrk_op<float>(U, T != 'N', *n, *k, *alpha, a, *lda, *beta, c, *ldc, zg::blas_stream(),
+                                            ZYNUM_GPU_MODE_PLAIN, 0, 0));
+}
+
+void dsyrk_(const char* u, const char* t, const zynum_blas_int* n, const zynum_blas_int* k, const double* alpha,
+            const double* a, const zynum_blas_int* lda, const double* beta, double* c, const zynum_blas_int* ldc) {
+    const char U = up(u), T = up(t);
+    if (report_error("DSYRK ", rank_k_error(U, T, *n, *k, *lda, *ldc))) return;
+    note_status("DSYRK", zg::syrk_op<double>(U, T != 'N', *n, *k, *alpha, a, *lda, *beta, c, *ldc,
+                                             zg::blas_stream(), ZYNUM_GPU_MODE_PLAIN, 0, 0));
+}
+
+void strsm_(const char* side, const char* u, const char* t, const char* d, const zynum_blas_int* m,
+            const zynum_blas_int* n, const float* alpha, const float* a, const zynum_blas_int* lda, float* b,
+            const zynum_blas_int* ldb) {
+    const char S = up(side), U = up(u), T = up(t), D = up(d);
+    if (report_error("STRSM ", triangular_error(S, U, T, D, *m, *n, *lda, *ldb))) return;
+    note_status("STRSM", zg::trsm_op<float>(S, U, T != 'N', D, *m, *n, *alpha, a, *lda, b, *ldb, zg::blas_stream(),
+                                            ZYNUM_GPU_MODE_PLAIN, 0, 0));
+}
+
+void dtrsm_(const char* side, const char* u, const char* t, const char* d, const zynum_blas_int* m,
+            const zynum_blas_int* n, const double* alpha, const double* a, const zynum_blas_int* lda, double* b,
+            const zynum_blas_int* ldb) {
+    const char S = up(side), U = up(u), T = up(t), D = up(d);
+    if (report_error("DTRSM ", triangular_error(S, U, T, D, *m, *n, *lda, *ldb))) return;
+    note_status("DTRSM", zg::trsm_op<double>(S, U, T != 'N', D, *m, *n, *alpha, a, *lda, b, *ldb, zg::blas_stream(),
+                                             ZYNUM_GPU_MODE_PLAIN, 0, 0));
+}
+
+// ---------------------------------------------------------------------------
+// CBLAS ABI (cblas.zig): invalid enums return silently with outputs
+// unchanged; numeric checks are the Fortran ones after the row-major swap.
+// ---------------------------------------------------------------------------
+static inline bool cb_layout(int l) { return l == CblasRowMajor || l == CblasColMajor; }
+static inline bool cb_trans(int t) { return t == CblasNoTrans || t == CblasTrans || t == CblasConjTrans; }
+static inline char cb_trans_char(int t) {
+    return t == CblasNoTrans ? 'N' : t == CblasTrans ? 'T' : t == CblasConjTrans ? 'C' : 'X';
+}
+static inline char cb_uplo_char(int u) { return u == CblasUpper ? 'U' : u == CblasLower ? 'L' : 'X'; }
+static inline char cb_row_uplo_char(int u) { return u == CblasUpper ? 'L' : u == CblasLower ? 'U' : 'X'; }
+static inline char cb_side_char(int s) { return s == CblasLeft ? 'L' : s == CblasRight ? 'R' : 'X'; }
+static inline char cb_row_side_char(int s) { return s == CblasLeft ? 'R' : s == CblasRight ? 'L' : 'X'; }
+static inline char cb_diag_char(int d) { return d == CblasNonUnit ? 'N' : d == CblasUnit ? 'U' : 'X'; }
+static inline char cb_row_trans_char(int t) {
+    return t == CblasNoTrans ? 'T' : t == CblasTrans ? 'N' : t == CblasConjTrans ? 'C' : 'X';
+}
+
+void cblas_sgemm(int layout, int transa, int transb, int m, int n, int k, float alpha, const float* a, int lda,
+                 const float* b, int ldb, float beta, float* c, int ldc) {
+    if (!cb_layout(layout) || !cb_trans(transa) || !cb_trans(transb)) return;
+    const bool row = layout == CblasRowMajor;
+    char ta = cb_trans_char(row ? transb : transa), tb = cb_trans_char(row ? transa : transb);
+    zynum_blas_int mm = row ? n : m, nn = row ? m : n, kk = k, la = row ? ldb : lda, lb = row ? lda : ldb, lc = ldc;
+    sgemm_(&ta, &tb, &mm, &nn, &kk, &alpha, row ? b : a, &la, row ? a : b, &lb, &beta, c, &lc);
+}
+
+void cblas_dgemm(int layout, int transa, int transb, int m, int n, int k, double alpha, const double* a, int lda,
+                 const double* b, int ldb, double beta, double* c, int ldc) {
+    if (!cb_layout(layout) || !cb_trans(transa) || !cb_trans(transb)) return;
+    const bool row = layout == CblasRowMajor;
+    char ta = cb_trans_char(row ? transb : transa), tb = cb_trans_char(row ? transa : transb);
+    zynum_blas_int mm = row ? n : m, nn = row ? m : n, kk = k, la = row ? ldb : lda, lb = row ? lda : ldb, lc = ldc;
+    dgemm_(&ta, &tb, &mm, &nn, &kk, &alpha, row ? b : a, &la, row ? a : b, &lb, &beta, c, &lc);
+}
+
+void cblas_dsyrk(int layout, int uplo, int trans, int n, int k, double alpha, const double* a, int lda, double beta,
+                 double* c, int ldc) {
+    if (!cb_layout(layout)) return;
+    const bool row = layout == CblasRowMajor;
+    char uu = row ? cb_row_uplo_char(uplo) : cb_uplo_char(uplo);
+    char tt = row ? cb_row_trans_char(trans) : cb_trans_char(trans);
+    zynum_blas_int nn = n, kk = k, la = lda, lc = ldc;
+    dsyrk_(&uu, &tt, &nn, &kk, &alpha, a, &la, &beta, c, &lc);
+}
+
+void cblas_dtrsm(int layout, int side, int uplo, int transa, int diag, int m, int n, double alpha, const double* a,
+                 int lda, double* b, int ldb) {
+    if (!cb_layout(layout) || !(side == CblasLeft || side == CblasRight) ||
+        !(uplo == CblasUpper || uplo == CblasLower) || !cb_trans(transa) ||
+        !(diag == CblasNonUnit || diag == CblasUnit))
+        return;
+    const bool row = layout == CblasRowMajor;
+    char ss = row ? cb_row_side_char(side) : cb_side_char(side);
+    char uu = row ? cb_row_uplo_char(uplo) : cb_uplo_char(uplo);
+    char tt = cb_trans_char(transa), dd = cb_diag_char(diag);
+    zynum_blas_int mm = row ? n : m, nn = row ? m : n, la = lda, lb = ldb;
+    dtrsm_(&ss, &uu, &tt, &dd, &mm, &nn, &alpha, a, &la, b, &lb);
+}
+
+// ---------------------------------------------------------------------------
+// GPU-tier private ABI
+// ---------------------------------------------------------------------------
+int zynum_gpu_gemm(const zynum_gpu_gemm_request* r) {
+    if (r == nullptr) return ZYNUM_GPU_DECLINE_ARGUMENT;
+    const char ta = r->transa == ZYNUM_GPU_NO_TRANS ? 'N' : r->transa <= ZYNUM_GPU_CONJ_TRANS ? 'T' : 'X';
+    const char tb = r->transb == ZYNUM_GPU_NO_TRANS ? 'N' : r->transb <= ZYNUM_GPU_CONJ_TRANS ? 'T' : 'X';
+    if (gemm_error(ta, tb, r->m, r->n, r->k, r->lda, r->ldb, r->ldc) != 0) return ZYNUM_GPU_DECLINE_ARGUMENT;
+    if (r->alpha_im != 0 || r->beta_im != 0) return ZYNUM_GPU_DECLINE_UNSUPPORTED;
+    hipStream_t s = r->stream ? reinterpret_cast<hipStream_t>(r->stream) : zg::blas_stream();
+    if (r->scalar == ZYNUM_GPU_F64) {
+        return zg::gemm_op<double>(ta == 'T', tb == 'T', r->m, r->n, r->k, scalar_from_bits<double>(r->alpha_re),
+                                   static_cast<const double*>(r->a), r->lda, static_cast<const double*>(r->b), r->ldb,
+                                   scalar_from_bits<double>(r->beta_re), static_cast<double*>(r->c), r->ldc, s,
+                                   r->mode, r->mode_param, r->flags);
+    }
+    if (r->scalar == ZYNUM_GPU_F32) {
+        return zg::gemm_op<float>(ta == 'T', tb == 'T', r->m, r->n, r->k, scalar_from_bits<float>(r->alpha_re),
+                                  static_cast<const float*>(r->a), r->lda, static_cast<const float*>(r->b), r->ldb,
+                                  scalar_from_bits<float>(r->beta_re), static_cast<float*>(r->c), r->ldc, s, r->mode,
+                                  r->mode_param, r->flags);
+    }
+    return ZYNUM_GPU_DECLINE_UNSUPPORTED;
+}
+
+int zynum_gpu_dtrsm(char side, char uplo, char transa, char diag, int m, int n, double alpha, const double* a,
+                    int lda, double* b, int ldb, void* stream, int mode, int mode_param, int flags) {
+    const char S = (char)toupper(side), U = (char)toupper(uplo), T = (char)toupper(transa), D = (char)toupper(diag);
+    if (triangular_error(S, U, T, D, m, n, lda, ldb) != 0) return ZYNUM_GPU_DECLINE_ARGUMENT;
+    hipStream_t s = stream ? reinterpret_cast<hipStream_t>(stream) : zg::blas_stream();
+    return zg::trsm_op<double>(S, U, T != 'N', D, m, n, alpha, a, lda, b, ldb, s, mode, mode_param, flags);
+}
+
+int zynum_gpu_dsyrk(char uplo, char trans, int n, int k, double alpha, const double* a, int lda, double beta,
+                    double* c, int ldc, void* stream, int mode, int mode_param, int flags) {
+    const char U = (char)toupper(uplo), T = (char)toupper(trans);
+    if (rank_k_error(U, T, n, k, lda, ldc) != 0) return ZYNUM_GPU_DECLINE_ARGUMENT;
+    hipStream_t s = stream ? reinterpret_cast<hipStream_t>(stream) : zg::blas_stream();
+    return zg::syrk_op<double>(U, T != 'N', n, k, alpha, a, lda, beta, c, ldc, s, mode, mode_param, flags);
+}
+
+int zynum_gpu_last_status(void) { return t_last_status; }
+
+const char* zynum_gpu_status_string(int status) {
+    switch (status) {
+        case ZYNUM_GPU_OK: return "ok";
+        case ZYNUM_GPU_DECLINE_NO_DEVICE: return "no gfx950 device";
+        case ZYNUM_GPU_DECLINE_ALLOC: return "device allocation failed";
+        case ZYNUM_GPU_DECLINE_UNSUPPORTED: return "unsupported form";
+        case ZYNUM_GPU_DECLINE_ARGUMENT: return "invalid argument";
+        case ZYNUM_GPU_DECLINE_FORCED: return "forced decline";
+        case ZYNUM_GPU_ERR_LAUNCH: return "HIP launch error";
+        default: return "unknown status";
+    }
+}
+
+const char* zynum_gpu_version(void) { return "zynum-gpu 0.1 gfx950"; }
+
+}  // extern "C"

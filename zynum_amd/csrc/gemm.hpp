@@ -1,0 +1,38 @@
+// Host-side entry points of the gfx950 GEMM core (device pointers only).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace zg {
+
+// One column-major GEMM: C = alpha * op(A) * op(B) + beta * C.
+template <typename T>
+struct GemmDesc {
+    bool ta = false, tb = false;
+    int m = 0, n = 0, k = 0;
+    int64_t lda = 1, ldb = 1, ldc = 1;
+    T alpha = T(1), beta = T(0);
+    const T* a = nullptr;
+    const T* b = nullptr;
+    T* c = nullptr;
+};
+
+// Tile configurations the launcher can pick (see gemm_mfma.hip).
+enum TileConfig : int {
+    kTileAuto = -1,
+    kTile128x128 = 0,  // 256 threads, 2x2 waves of 64x64
+    kTile64x64 = 1,    // 256 threads, 2x2 waves of 32x32
+    kTile256x128 = 2,  // 256 threads, 2x2 waves of 128x64
+    kTileConfigs = 3
+};
+
+// Launch the plain MFMA GEMM (FP64 or FP32). Returns a hipError_t.
+template <typename T>
+hipError_t gemm_launch(const GemmDesc<T>& d, hipStream_t stream, int tile = kTileAuto);
+
+// C = beta * C (beta == 0 writes exact zeros), the k == 0 / alpha == 0 path.
+template <typename T>
+hipError_t scale_launch(int m, int n, T beta, T* c, int64_t ldc, hipStream_t stream);
+
+}  // namespace zg

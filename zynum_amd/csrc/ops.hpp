@@ -1,0 +1,36 @@
+// Checked, residency-aware operations behind the C ABI.  Each returns a
+// ZYNUM_GPU_* code; nonzero codes other than ZYNUM_GPU_ERR_LAUNCH are
+// returned before any caller-visible memory is written.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace zg {
+
+template <typename T>
+int gemm_op(bool ta, bool tb, int m, int n, int k, T alpha, const T* a, int64_t lda, const T* b, int64_t ldb,
+            T beta, T* c, int64_t ldc, hipStream_t s, int mode, int mode_param, int flags);
+
+template <typename T>
+int trsm_op(char side, char uplo, bool trans, char diag, int m, int n, T alpha, const T* a, int64_t lda, T* b,
+            int64_t ldb, hipStream_t s, int mode, int mode_param, int flags);
+
+template <typename T>
+int syrk_op(char uplo, bool trans, int n, int k, T alpha, const T* a, int64_t lda, T beta, T* c, int64_t ldc,
+            hipStream_t s, int mode, int mode_param, int flags);
+
+// Device-pointer implementations (no staging), stream ordered.
+int ozaki_dgemm(bool ta, bool tb, int m, int n, int k, double alpha, const double* a, int64_t lda,
+                const double* b, int64_t ldb, double beta, double* c, int64_t ldc, hipStream_t s, int mode,
+                int mode_param);
+int dd_dgemm(bool ta, bool tb, int m, int n, int k, double alpha, const double* a, int64_t lda, const double* b,
+             int64_t ldb, double beta, double* c, int64_t ldc, hipStream_t s);
+template <typename T>
+int trsm_device(char side, char uplo, bool trans, char diag, int m, int n, T alpha, const T* a, int64_t lda, T* b,
+                int64_t ldb, hipStream_t s, int mode, int mode_param);
+template <typename T>
+int syrk_device(char uplo, bool trans, int n, int k, T alpha, const T* a, int64_t lda, T beta, T* c, int64_t ldc,
+                hipStream_t s, int mode, int mode_param);
+
+}  // namespace zg

@@ -1,0 +1,61 @@
+// Host runtime of the GPU tier: device checks, stream choice, operand
+// residency, staging of host operands and the shared device workspace.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include "../../include/zynum_gpu.h"
+
+namespace zg {
+
+// True when a gfx950 device is present and current (probed once per process).
+bool device_ready();
+
+// Stream used by the classic BLAS symbols (set by zynum_gpu_set_stream).
+hipStream_t blas_stream();
+
+// Consumes one injected decline; true if the call must decline.
+bool take_forced_decline();
+
+// Residency of a pointer: true for device or managed memory.
+bool is_device_pointer(const void* p);
+
+// Elements spanned by a column-major rows x cols matrix with leading dim ld.
+inline size_t matrix_extent(int64_t rows, int64_t cols, int64_t ld) {
+    if (rows <= 0 || cols <= 0) return 0;
+    return (size_t)(rows + (cols - 1) * ld);
+}
+
+// A device view of a caller matrix.  Host matrices are copied to a device
+// buffer (only the logical rows x cols part) and, for outputs, copied back by
+// finish().  Device matrices are used in place.
+struct Operand {
+    void* dev = nullptr;       // pointer handed to kernels
+    void* owned = nullptr;     // staging allocation (host operand only)
+    const void* host = nullptr;
+    void* host_out = nullptr;
+    int64_t rows = 0, cols = 0, ld = 0;
+    size_t elem = 0;
+    bool staged = false;
+
+    // Prepare an input (copied in when staged).
+    int bind_in(const void* p, int64_t r, int64_t c, int64_t ldx, size_t es, hipStream_t s);
+    // Prepare an output; `copy_in` when its old values are read (beta != 0).
+    int bind_out(void* p, int64_t r, int64_t c, int64_t ldx, size_t es, bool copy_in, hipStream_t s);
+    // Copy a staged output back to the host.
+    int finish(hipStream_t s);
+    void release();
+    ~Operand() { release(); }
+};
+
+// Shared grow-only device workspace (one per process, current device).  The
+// lock is held by the caller for the whole stream-ordered use.
+void* workspace_acquire(size_t bytes);  // returns nullptr on allocation failure; holds the lock
+void workspace_release();
+void workspace_free_all();
+
+int status_from_hip(hipError_t e);
+
+}  // namespace zg
