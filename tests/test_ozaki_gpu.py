@@ -1,0 +1,150 @@
+"""Split-precision DGEMM (Ozaki scheme I with INT8 slices, Ozaki scheme II
+with INT8 residues + CRT) and the on-device double-double reference.
+
+Bars: bitwise on integer-valued inputs (scaling, slicing and residues are
+exact there, and the CRT recovers the exact integer product); on random
+inputs, the method's own error bound against the double-double value, and
+"no worse than plain FP64 DGEMM" on the wide-range workload.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from conftest import int_gemm
+from test_oracle import operands
+
+pytestmark = pytest.mark.gpu
+
+MODULI = [256, 255, 253, 251, 247, 241, 239, 233, 229, 227, 223, 217, 211, 199, 197, 193, 191, 181, 179, 173]
+
+
+def run(torch, ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, mode, param):
+    from zynum_amd import _lib, blas
+
+    da, db, dc = (torch.from_numpy(x.copy()).cuda() for x in (a, b, c))
+    st = blas.gemm_request(ta != "N", tb != "N", m, n, k, alpha, da.data_ptr(), lda, db.data_ptr(), ldb, beta,
+                           dc.data_ptr(), ldc, _lib.F64, mode=mode, mode_param=param)
+    _lib.check(st, f"mode {mode}/{param}")
+    torch.cuda.synchronize()
+    return dc.cpu().numpy()
+
+
+def dd_ref(torch, ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc):
+    from zynum_amd import _lib
+
+    lib = _lib.load()
+    da, db, dc = (torch.from_numpy(x.copy()).cuda() for x in (a, b, c))
+    hi = torch.zeros(ldc * n, dtype=torch.float64, device="cuda")
+    lo = torch.zeros(ldc * n, dtype=torch.float64, device="cuda")
+    st = lib.zynum_gpu_dd_gemm(ta.encode(), tb.encode(), m, n, k, alpha, da.data_ptr(), lda, db.data_ptr(), ldb,
+                               beta, dc.data_ptr(), ldc, hi.data_ptr(), lo.data_ptr(), ldc, None)
+    _lib.check(st, "dd")
+    return hi.cpu().numpy(), lo.cpu().numpy()
+
+
+def logical(x, rows, cols, ld):
+    return x[: ld * cols].reshape(cols, ld)[:, :rows].T
+
+
+def test_dd_gemm_matches_cpu_double_double(torch_cuda, oracle):
+    """The device double-double product equals the CPU double-double oracle to
+    ~2^-100 relative on sampled entries (both use TwoProd/TwoSum)."""
+    rng = np.random.default_rng(9)
+    m, n, k = 70, 53, 600
+    a = rng.uniform(-1, 1, m * k) * np.exp2(rng.integers(-16, 17, m * k))
+    b = rng.uniform(-1, 1, k * n) * np.exp2(rng.integers(-16, 17, k * n))
+    c = rng.uniform(-1, 1, m * n)
+    hi, lo = dd_ref(torch_cuda, "N", "N", m, n, k, 1.5, a, m, b, k, 0.25, c, m)
+    rows = rng.integers(0, m, 200)
+    cols = rng.integers(0, n, 200)
+    chi, clo = oracle.dd_gemm_entries("N", "N", k, 1.5, a, m, b, k, 0.25, c, m, rows, cols)
+    idx = rows + cols * m
+    mag = np.abs(logical(a.reshape(-1), m, k, m)) @ np.abs(logical(b.reshape(-1), k, n, k))
+    diff = np.abs((hi[idx] - chi) + (lo[idx] - clo))
+    assert np.all(diff <= 2.0**-98 * (1.5 * mag[rows, cols] + 1))
+
+
+@pytest.mark.parametrize("mode,param", [(3, 14), (3, 8), (3, 20), (1, 2), (1, 4), (1, 7)])
+@pytest.mark.parametrize("ta,tb", [("N", "N"), ("N", "T"), ("T", "N"), ("T", "T")])
+def test_integer_inputs_exact(torch_cuda, mode, param, ta, tb):
+    """Integer inputs in [-14, 14]: power-of-two scaling, slicing and residues
+    are exact, the CRT recovers the exact integer product, so the result must
+    equal exact arithmetic bitwise (odd sizes, padded ld, alpha/beta)."""
+    m, n, k = 131, 77, 203
+    lda = (m if ta == "N" else k) + 5
+    ldb = (k if tb == "N" else n) + 3
+    ldc = m + 2
+    a, b, c = operands(ta, tb, m, n, k, lda, ldb, ldc, np.float64, True, salts=(3, 8, 13))
+    want = int_gemm(ta, tb, m, n, k, 0.75, a, lda, b, ldb, -0.25, c, ldc)
+    got = run(torch_cuda, ta, tb, m, n, k, 0.75, a, lda, b, ldb, -0.25, c, ldc, mode, param)
+    assert np.array_equal(got, want)
+
+
+def crt_alpha(nmod: int, k: int) -> int:
+    log2p = sum(math.log2(x) for x in MODULI[:nmod])
+    alpha = math.floor((log2p - 1) / 2)
+    while 2 * math.log2(2.0**alpha + 0.5 * math.sqrt(k)) > log2p - 1 - 1e-6:
+        alpha -= 1
+    return alpha
+
+
+@pytest.mark.parametrize("nmod", [10, 14, 16])
+def test_crt_error_bound_random(torch_cuda, nmod):
+    """Ozaki II error bound: |C - C_exact| <= 4 sqrt(K) 2^-alpha ||a_i|| ||b_j||
+    (rounding a, b to integers after scaling their norms to 2^alpha) plus one
+    rounding of the result, checked against the double-double product."""
+    rng = np.random.default_rng(nmod)
+    m, n, k = 192, 160, 1000
+    a = rng.uniform(-1, 1, m * k) * np.exp2(rng.integers(-16, 17, m * k))
+    b = rng.uniform(-1, 1, k * n) * np.exp2(rng.integers(-16, 17, k * n))
+    c = np.zeros(m * n)
+    got = run(torch_cuda, "N", "N", m, n, k, 1.0, a, m, b, k, 0.0, c, m, 3, nmod)
+    hi, lo = dd_ref(torch_cuda, "N", "N", m, n, k, 1.0, a, m, b, k, 0.0, c, m)
+    err = np.abs((got - hi) - lo)
+    A = a.reshape(k, m).T
+    B = b.reshape(n, k).T
+    na = np.linalg.norm(A, axis=1)
+    nb = np.linalg.norm(B, axis=0)
+    alpha = crt_alpha(nmod, k)
+    bound = 4 * math.sqrt(k) * 2.0**-alpha * np.outer(na, nb).ravel(order="F") + 2.0**-52 * np.abs(hi)
+    assert np.all(err <= bound)
+
+
+def test_wide_range_no_worse_than_plain_fp64(torch_cuda):
+    """On the wide-range workload shape (a = u 2^e, e in [-16, 16]) the 14-residue
+    CRT form's max error against double-double is no larger than the plain FP64
+    MFMA DGEMM's; 7-slice Ozaki I stays within 4x of it."""
+    from zynum_amd import synth
+
+    n = 1024
+    a = synth.wide_range(7, 0, n, n, n)
+    b = synth.wide_range(7, n * n, n, n, n)
+    c = np.zeros(n * n)
+    hi, lo = dd_ref(torch_cuda, "N", "N", n, n, n, 1.0, a, n, b, n, 0.0, c, n)
+    plain = run(torch_cuda, "N", "N", n, n, n, 1.0, a, n, b, n, 0.0, c, n, 0, 0)
+    crt = run(torch_cuda, "N", "N", n, n, n, 1.0, a, n, b, n, 0.0, c, n, 3, 14)
+    sl7 = run(torch_cuda, "N", "N", n, n, n, 1.0, a, n, b, n, 0.0, c, n, 1, 7)
+    e_plain = np.max(np.abs((plain - hi) - lo))
+    e_crt = np.max(np.abs((crt - hi) - lo))
+    e_sl7 = np.max(np.abs((sl7 - hi) - lo))
+    print(f"max err vs dd: plain {e_plain:.3e}  crt14 {e_crt:.3e}  slices7 {e_sl7:.3e}")
+    assert e_crt <= e_plain
+    assert e_sl7 <= 4 * e_plain
+
+
+def test_slices_error_decreases_with_slices(torch_cuda):
+    """Ozaki I: each extra 7-bit slice cuts the error by roughly 2^7."""
+    rng = np.random.default_rng(4)
+    m = n = 128
+    k = 512
+    a = rng.uniform(-1, 1, m * k)
+    b = rng.uniform(-1, 1, k * n)
+    c = np.zeros(m * n)
+    hi, lo = dd_ref(torch_cuda, "N", "N", m, n, k, 1.0, a, m, b, k, 0.0, c, m)
+    errs = []
+    for s in (2, 3, 4, 5):
+        got = run(torch_cuda, "N", "N", m, n, k, 1.0, a, m, b, k, 0.0, c, m, 1, s)
+        errs.append(np.max(np.abs((got - hi) - lo)))
+    for e0, e1 in zip(errs, errs[1:]):
+        assert e1 < e0 / 16

@@ -41,7 +41,8 @@ class GemmRequest(C.Structure):
         ("alpha_re", C.c_uint64), ("alpha_im", C.c_uint64), ("beta_re", C.c_uint64), ("beta_im", C.c_uint64),
         ("a", C.c_void_p), ("b", C.c_void_p), ("c", C.c_void_p),
         ("stream", C.c_void_p),
-        ("mode", C.c_uint8), ("mode_param", C.c_uint8), ("flags", C.c_uint8), ("reserved1", C.c_uint8 * 5),
+        ("mode", C.c_uint8), ("mode_param", C.c_uint8), ("flags", C.c_uint8), ("tile", C.c_uint8),
+        ("reserved1", C.c_uint8 * 4),
     ]
 
 

@@ -163,7 +163,7 @@ def _bits(value: float, scalar: int) -> int:
 
 def gemm_request(transa: bool, transb: bool, m: int, n: int, k: int, alpha: float, a: int, lda: int,
                  b: int, ldb: int, beta: float, c: int, ldc: int, scalar: int, stream: int | None = None,
-                 mode: int = _lib.MODE_PLAIN, mode_param: int = 0, flags: int = 0) -> int:
+                 mode: int = _lib.MODE_PLAIN, mode_param: int = 0, flags: int = 0, tile: int = 0) -> int:
     """Fill and submit a ``zynum_gpu_gemm_request``; returns its status code."""
     r = _lib.GemmRequest()
     r.transa = _lib.TRANS if transa else _lib.NO_TRANS
@@ -174,7 +174,7 @@ def gemm_request(transa: bool, transb: bool, m: int, n: int, k: int, alpha: floa
     r.beta_re = _bits(beta, scalar)
     r.a, r.b, r.c = a, b, c
     r.stream = stream
-    r.mode, r.mode_param, r.flags = mode, mode_param, flags
+    r.mode, r.mode_param, r.flags, r.tile = mode, mode_param, flags, tile
     return _lib.load().zynum_gpu_gemm(C.byref(r))
 
 

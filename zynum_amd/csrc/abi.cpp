@@ -105,7 +105,7 @@ namespace zg {
 
 template <typename T>
 int gemm_op(bool ta, bool tb, int m, int n, int k, T alpha, const T* a, int64_t lda, const T* b, int64_t ldb,
-            T beta, T* c, int64_t ldc, hipStream_t s, int mode, int mode_param, int flags) {
+            T beta, T* c, int64_t ldc, hipStream_t s, int mode, int mode_param, int flags, int tile) {
     if (m <= 0 || n <= 0) return ZYNUM_GPU_OK;  // gemm.zig:1393
     if (take_forced_decline()) return ZYNUM_GPU_DECLINE_FORCED;
     if (!device_ready()) return ZYNUM_GPU_DECLINE_NO_DEVICE;
@@ -131,7 +131,7 @@ int gemm_op(bool ta, bool tb, int m, int n, int k, T alpha, const T* a, int64_t 
         d.lda = lda; d.ldb = ldb; d.ldc = ldc;
         d.alpha = alpha; d.beta = beta;
         d.a = da; d.b = db; d.c = dc;
-        st = status_from_hip(gemm_launch<T>(d, s));
+        st = status_from_hip(gemm_launch<T>(d, s, tile));
     } else if (mode == ZYNUM_GPU_MODE_DD) {
         if constexpr (sizeof(T) == 8) st = dd_dgemm(ta, tb, m, n, k, alpha, da, lda, db, ldb, beta, dc, ldc, s);
     } else {
@@ -186,9 +186,9 @@ int syrk_op(char uplo, bool trans, int n, int k, T alpha, const T* a, int64_t ld
 }
 
 template int gemm_op<double>(bool, bool, int, int, int, double, const double*, int64_t, const double*, int64_t,
-                             double, double*, int64_t, hipStream_t, int, int, int);
+                             double, double*, int64_t, hipStream_t, int, int, int, int);
 template int gemm_op<float>(bool, bool, int, int, int, float, const float*, int64_t, const float*, int64_t, float,
-                            float*, int64_t, hipStream_t, int, int, int);
+                            float*, int64_t, hipStream_t, int, int, int, int);
 
 }  // namespace zg
 
@@ -340,18 +340,19 @@ int zynum_gpu_gemm(const zynum_gpu_gemm_request* r) {
     const char tb = r->transb == ZYNUM_GPU_NO_TRANS ? 'N' : r->transb <= ZYNUM_GPU_CONJ_TRANS ? 'T' : 'X';
     if (gemm_error(ta, tb, r->m, r->n, r->k, r->lda, r->ldb, r->ldc) != 0) return ZYNUM_GPU_DECLINE_ARGUMENT;
     if (r->alpha_im != 0 || r->beta_im != 0) return ZYNUM_GPU_DECLINE_UNSUPPORTED;
+    if (r->tile > zg::kTileConfigs) return ZYNUM_GPU_DECLINE_UNSUPPORTED;
     hipStream_t s = r->stream ? reinterpret_cast<hipStream_t>(r->stream) : zg::blas_stream();
     if (r->scalar == ZYNUM_GPU_F64) {
         return zg::gemm_op<double>(ta == 'T', tb == 'T', r->m, r->n, r->k, scalar_from_bits<double>(r->alpha_re),
                                    static_cast<const double*>(r->a), r->lda, static_cast<const double*>(r->b), r->ldb,
                                    scalar_from_bits<double>(r->beta_re), static_cast<double*>(r->c), r->ldc, s,
-                                   r->mode, r->mode_param, r->flags);
+                                   r->mode, r->mode_param, r->flags, (int)r->tile - 1);
     }
     if (r->scalar == ZYNUM_GPU_F32) {
         return zg::gemm_op<float>(ta == 'T', tb == 'T', r->m, r->n, r->k, scalar_from_bits<float>(r->alpha_re),
                                   static_cast<const float*>(r->a), r->lda, static_cast<const float*>(r->b), r->ldb,
                                   scalar_from_bits<float>(r->beta_re), static_cast<float*>(r->c), r->ldc, s, r->mode,
-                                  r->mode_param, r->flags);
+                                  r->mode_param, r->flags, (int)r->tile - 1);
     }
     return ZYNUM_GPU_DECLINE_UNSUPPORTED;
 }

@@ -15,10 +15,16 @@ constexpr int kWave = 64;
 // 16x16x4 MFMA for the two real scalar types.  Operand layout (both types):
 //   A (16x4):  lane l holds A[l % 16][l / 16]
 //   B (4x16):  lane l holds B[l / 16][l % 16]
-//   D (16x16): lane l, item r holds D[4 * (l / 16) + r][l % 16]
+// Result layout differs:
+//   f32: lane l, item r holds D[4 * (l / 16) + r][l % 16]
+//   f64: lane l, item r holds D[4 * r + l / 16][l % 16]
+// a_row(i) is the A row a lane with l % 16 == i feeds.  For f64 it permutes
+// the rows (i -> 4 (i % 4) + i / 4) so that the f64 result also lands as four
+// consecutive rows per lane, like f32; the epilogue is then shared.
 template <typename T> struct Mfma;
 template <> struct Mfma<double> {
     typedef v4d acc_t;
+    static __device__ __forceinline__ int a_row(int i) { return 4 * (i & 3) + (i >> 2); }
     static __device__ __forceinline__ acc_t zero() { return acc_t{0.0, 0.0, 0.0, 0.0}; }
     static __device__ __forceinline__ acc_t mma(double a, double b, acc_t c) {
         return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
@@ -26,6 +32,7 @@ template <> struct Mfma<double> {
 };
 template <> struct Mfma<float> {
     typedef v4f acc_t;
+    static __device__ __forceinline__ int a_row(int i) { return i; }
     static __device__ __forceinline__ acc_t zero() { return acc_t{0.f, 0.f, 0.f, 0.f}; }
     static __device__ __forceinline__ acc_t mma(float a, float b, acc_t c) {
         return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);

@@ -21,11 +21,20 @@ struct GemmDesc {
 // Tile configurations the launcher can pick (see gemm_mfma.hip).
 enum TileConfig : int {
     kTileAuto = -1,
-    kTile128x128 = 0,  // 256 threads, 2x2 waves of 64x64
-    kTile64x64 = 1,    // 256 threads, 2x2 waves of 32x32
-    kTile256x128 = 2,  // 256 threads, 2x2 waves of 128x64
-    kTileConfigs = 3
+    kTile64x64 = 0,       // 256 threads, 2x2 waves of 32x32, BK 16
+    kTile128x128 = 1,     // 256 threads, 2x2 waves of 64x64, BK 16
+    kTile128x128o2 = 2,   // as 1, registers capped for 2 waves per SIMD
+    kTile128x64 = 3,      // 2x2 waves of 64x32, 2 waves per SIMD
+    kTile64x128 = 4,      // 2x2 waves of 32x64, 2 waves per SIMD
+    kTile128x128k32 = 5,  // 2x2 waves of 64x64, BK 32
+    kTile128x128w8 = 6,   // 512 threads, 4x2 waves of 32x64
+    kTile64x64k32 = 7,    // as 0 with BK 32
+    kTileConfigs = 8
 };
+
+// Tile choice for one GEMM shape from the measured tuning table
+// (tune_table.inc, written by zynum_amd/autotune.py).
+int pick_tile(int m, int n, int k, int elem_bytes);
 
 // Launch the plain MFMA GEMM (FP64 or FP32). Returns a hipError_t.
 template <typename T>

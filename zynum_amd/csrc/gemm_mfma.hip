@@ -104,12 +104,23 @@ struct Stage {
     }
 };
 
-template <typename T, int BM, int BN, int BK, bool TA, bool TB, int VEC>
-__global__ __launch_bounds__(256) void gemm_mfma_kernel(GemmArgs<T> p) {
-    constexpr int NT = 256;
+// Tile configuration: BM x BN tile per workgroup, BK-deep K slices, WM x WN
+// wave grid (each wave owns (BM/WM) x (BN/WN)), OCC = waves per SIMD the
+// register budget is sized for (0 = compiler's choice).
+template <int BM_, int BN_, int BK_, int WM_, int WN_, int OCC_>
+struct TileCfg {
+    static constexpr int BM = BM_, BN = BN_, BK = BK_, WM = WM_, WN = WN_, OCC = OCC_;
+    static constexpr int NT = 64 * WM_ * WN_;
+};
+
+template <typename T, class CF, bool TA, bool TB, int VEC>
+__global__ __launch_bounds__(CF::NT) __attribute__((amdgpu_waves_per_eu(CF::OCC > 0 ? CF::OCC : 1)))
+void gemm_mfma_kernel(GemmArgs<T> p) {
+    constexpr int BM = CF::BM, BN = CF::BN, BK = CF::BK, NT = CF::NT;
     constexpr int PAD = 16 / sizeof(T);
-    constexpr int WTM = BM / 2, WTN = BN / 2;
+    constexpr int WTM = BM / CF::WM, WTN = BN / CF::WN;
     constexpr int FM = WTM / 16, FN = WTN / 16;
+    static_assert(FM >= 1 && FN >= 1, "wave tile smaller than one MFMA block");
     using M = Mfma<T>;
     using Acc = typename M::acc_t;
 
@@ -119,8 +130,8 @@ __global__ __launch_bounds__(256) void gemm_mfma_kernel(GemmArgs<T> p) {
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wave = tid >> 6;
-    const int wm0 = (wave & 1) * WTM;
-    const int wn0 = (wave >> 1) * WTN;
+    const int wm0 = (wave % CF::WM) * WTM;
+    const int wn0 = (wave / CF::WM) * WTN;
 
     const int total = p.tiles_m * p.tiles_n;
     int tm, tn;
@@ -147,8 +158,9 @@ __global__ __launch_bounds__(256) void gemm_mfma_kernel(GemmArgs<T> p) {
     sb.store(lds_b[0], tid);
     __syncthreads();
 
-    const int lr = lane & 15;  // row within a 16x16 block (A) / column (B)
+    const int lr = lane & 15;  // column within a 16x16 block (B, D)
     const int lk = lane >> 4;  // k within a 4-deep step
+    const int ar = M::a_row(lr);  // A row fed by this lane
 
     for (int t = 0; t < nk; ++t) {
         const int cur = t & 1;
@@ -165,7 +177,7 @@ __global__ __launch_bounds__(256) void gemm_mfma_kernel(GemmArgs<T> p) {
         for (int kk = 0; kk < BK; kk += 4) {
             T af[FM], bf[FN];
 #pragma unroll
-            for (int i = 0; i < FM; ++i) af[i] = la[(kk + lk) * (BM + PAD) + wm0 + i * 16 + lr];
+            for (int i = 0; i < FM; ++i) af[i] = la[(kk + lk) * (BM + PAD) + wm0 + i * 16 + ar];
 #pragma unroll
             for (int j = 0; j < FN; ++j) bf[j] = lb[(kk + lk) * (BN + PAD) + wn0 + j * 16 + lr];
 #pragma unroll
@@ -223,48 +235,57 @@ hipError_t scale_launch(int m, int n, T beta, T* c, int64_t ldc, hipStream_t str
 // ---------------------------------------------------------------------------
 // dispatch
 // ---------------------------------------------------------------------------
-template <typename T, int BM, int BN, int BK, int VEC>
+template <typename T, class CF, int VEC>
 static hipError_t launch_cfg(const GemmDesc<T>& d, hipStream_t stream) {
     GemmArgs<T> p;
     p.m = d.m; p.n = d.n; p.k = d.k;
     p.lda = d.lda; p.ldb = d.ldb; p.ldc = d.ldc;
     p.alpha = d.alpha; p.beta = d.beta;
     p.a = d.a; p.b = d.b; p.c = d.c;
-    p.tiles_m = (d.m + BM - 1) / BM;
-    p.tiles_n = (d.n + BN - 1) / BN;
+    p.tiles_m = (d.m + CF::BM - 1) / CF::BM;
+    p.tiles_n = (d.n + CF::BN - 1) / CF::BN;
     p.group_m = 8;
     const int grid = p.tiles_m * p.tiles_n;
-    if (!d.ta && !d.tb) gemm_mfma_kernel<T, BM, BN, BK, false, false, VEC><<<grid, 256, 0, stream>>>(p);
-    else if (!d.ta && d.tb) gemm_mfma_kernel<T, BM, BN, BK, false, true, VEC><<<grid, 256, 0, stream>>>(p);
-    else if (d.ta && !d.tb) gemm_mfma_kernel<T, BM, BN, BK, true, false, VEC><<<grid, 256, 0, stream>>>(p);
-    else gemm_mfma_kernel<T, BM, BN, BK, true, true, VEC><<<grid, 256, 0, stream>>>(p);
+    if (!d.ta && !d.tb) gemm_mfma_kernel<T, CF, false, false, VEC><<<grid, CF::NT, 0, stream>>>(p);
+    else if (!d.ta && d.tb) gemm_mfma_kernel<T, CF, false, true, VEC><<<grid, CF::NT, 0, stream>>>(p);
+    else if (d.ta && !d.tb) gemm_mfma_kernel<T, CF, true, false, VEC><<<grid, CF::NT, 0, stream>>>(p);
+    else gemm_mfma_kernel<T, CF, true, true, VEC><<<grid, CF::NT, 0, stream>>>(p);
     return hipGetLastError();
 }
 
-template <typename T, int BM, int BN, int BK>
+template <typename T, class CF>
 static hipError_t launch_vec(const GemmDesc<T>& d, hipStream_t stream) {
     constexpr int V = 16 / sizeof(T);
     const uintptr_t mask = 15;
     const bool aligned = ((reinterpret_cast<uintptr_t>(d.a) & mask) == 0) &&
                          ((reinterpret_cast<uintptr_t>(d.b) & mask) == 0) && (d.lda % V == 0) &&
                          (d.ldb % V == 0);
-    if (aligned) return launch_cfg<T, BM, BN, BK, V>(d, stream);
-    return launch_cfg<T, BM, BN, BK, 1>(d, stream);
+    if (aligned) return launch_cfg<T, CF, V>(d, stream);
+    return launch_cfg<T, CF, 1>(d, stream);
+}
+
+// BK is given for FP64; FP32 uses twice the depth (same bytes per slice).
+template <typename T, int BM, int BN, int BK64, int WM, int WN, int OCC>
+static hipError_t launch_shape(const GemmDesc<T>& d, hipStream_t stream) {
+    constexpr int BK = (sizeof(T) == 8) ? BK64 : 2 * BK64;
+    return launch_vec<T, TileCfg<BM, BN, BK, WM, WN, OCC>>(d, stream);
 }
 
 template <typename T>
 hipError_t gemm_launch(const GemmDesc<T>& d, hipStream_t stream, int tile) {
     if (d.m <= 0 || d.n <= 0) return hipSuccess;
     if (d.k <= 0 || d.alpha == T(0)) return scale_launch<T>(d.m, d.n, d.beta, d.c, d.ldc, stream);
-    if (tile == kTileAuto) {
-        const int64_t tiles128 = (int64_t)((d.m + 127) / 128) * ((d.n + 127) / 128);
-        tile = (tiles128 >= 512) ? kTile128x128 : kTile64x64;
-    }
-    constexpr int BK = (sizeof(T) == 8) ? 16 : 32;
+    if (tile == kTileAuto) tile = pick_tile(d.m, d.n, d.k, (int)sizeof(T));
     switch (tile) {
-        case kTile64x64: return launch_vec<T, 64, 64, BK>(d, stream);
-        case kTile256x128: return launch_vec<T, 256, 128, (sizeof(T) == 8) ? 8 : 16>(d, stream);
-        default: return launch_vec<T, 128, 128, BK>(d, stream);
+        case kTile64x64: return launch_shape<T, 64, 64, 16, 2, 2, 0>(d, stream);
+        case kTile128x128: return launch_shape<T, 128, 128, 16, 2, 2, 1>(d, stream);
+        case kTile128x128o2: return launch_shape<T, 128, 128, 16, 2, 2, 2>(d, stream);
+        case kTile128x64: return launch_shape<T, 128, 64, 16, 2, 2, 2>(d, stream);
+        case kTile64x128: return launch_shape<T, 64, 128, 16, 2, 2, 2>(d, stream);
+        case kTile128x128k32: return launch_shape<T, 128, 128, 32, 2, 2, 1>(d, stream);
+        case kTile128x128w8: return launch_shape<T, 128, 128, 16, 4, 2, 2>(d, stream);
+        case kTile64x64k32: return launch_shape<T, 64, 64, 32, 2, 2, 0>(d, stream);
+        default: return hipErrorInvalidValue;
     }
 }
 

@@ -10,7 +10,7 @@ namespace zg {
 
 template <typename T>
 int gemm_op(bool ta, bool tb, int m, int n, int k, T alpha, const T* a, int64_t lda, const T* b, int64_t ldb,
-            T beta, T* c, int64_t ldc, hipStream_t s, int mode, int mode_param, int flags);
+            T beta, T* c, int64_t ldc, hipStream_t s, int mode, int mode_param, int flags, int tile = -1);
 
 template <typename T>
 int trsm_op(char side, char uplo, bool trans, char diag, int m, int n, T alpha, const T* a, int64_t lda, T* b,

@@ -1,11 +1,696 @@
-// Split-precision DGEMM (filled in after the plain path).
+// Split-precision DGEMM on gfx950 INT8 matrix cores.
+//
+// Two forms share one batched INT8 MFMA GEMM (v_mfma_i32_16x16x64_i8, exact
+// INT32 accumulation) over K-contiguous int8 "planes" of op(A) and op(B):
+//
+//  * Ozaki scheme II (mode OZAKI_CRT): every row of op(A) / column of op(B) is
+//    scaled by a power of two so its 2-norm is at most 2^alpha, rounded to an
+//    integer (a', b'), and reduced modulo N pairwise-coprime moduli m_l <= 256.
+//    The N residue products A'_l B'_l are exact in INT32; their epilogue
+//    stores u_l = (y_l * (A'_l B'_l mod m_l)) mod m_l as one byte, with
+//    y_l = (P / m_l)^-1 mod m_l.  The reconstruction kernel rebuilds
+//    C' = A'B' exactly from the u_l by the Chinese remainder theorem in FP64
+//    digits of 40 bits (every partial sum an exact integer), then applies the
+//    inverse scaling and the alpha/beta epilogue.  By Cauchy-Schwarz
+//    |C'_ij| <= ||a'_i|| ||b'_j|| < P/2, so C' is recovered exactly and the
+//    only error is the rounding of a and b to 2^-alpha of their row/column
+//    norms.  (Ozaki, Uchino, Imamura: "Ozaki Scheme II: a GEMM-oriented
+//    emulation of floating-point matrix multiplication using an integer
+//    modular technique", 2025.)
+//  * Ozaki scheme I (mode OZAKI_INT8): rows/columns are scaled by their max
+//    exponent and split into s signed slices of 7 bits (round to nearest);
+//    products of slice pairs with i + j <= s + 1 are grouped by level
+//    (one INT32 GEMM over the concatenated pairs per level) and accumulated
+//    level by level into an FP64 buffer.  (Ozaki, Ogita, Oishi, Rump 2012;
+//    Ootomo, Ozaki, Yokota 2024 for the INT8 form.)
+//
+// FP64 is used only for the scaling exponents, the residue / slice
+// extraction, the CRT reconstruction and the final level sums.
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <vector>
+
 #include "common.hpp"
+#include "gemm.hpp"
 #include "ops.hpp"
+#include "ozaki.hpp"
 #include "runtime.hpp"
 
 namespace zg {
-int ozaki_dgemm(bool, bool, int, int, int, double, const double*, int64_t, const double*, int64_t, double, double*,
-                int64_t, hipStream_t, int, int) {
+
+// Pairwise-coprime moduli, largest first (256 = 2^8, 255 = 3*5*17,
+// 253 = 11*23, 247 = 13*19, 217 = 7*31, the rest prime).
+static const int kModuli[kMaxModuli] = {256, 255, 253, 251, 247, 241, 239, 233, 229, 227,
+                                        223, 217, 211, 199, 197, 193, 191, 181, 179, 173};
+
+// ---------------------------------------------------------------------------
+// host bignum helpers (little-endian base 2^32), enough for P < 2^170
+// ---------------------------------------------------------------------------
+namespace {
+struct Big {
+    uint32_t w[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+};
+void big_mul_small(Big& x, uint32_t s) {
+    uint64_t carry = 0;
+    for (int i = 0; i < 8; ++i) {
+        uint64_t v = (uint64_t)x.w[i] * s + carry;
+        x.w[i] = (uint32_t)v;
+        carry = v >> 32;
+    }
+}
+uint32_t big_div_small(Big& x, uint32_t d) {  // x /= d, returns remainder
+    uint64_t rem = 0;
+    for (int i = 7; i >= 0; --i) {
+        uint64_t cur = (rem << 32) | x.w[i];
+        x.w[i] = (uint32_t)(cur / d);
+        rem = cur % d;
+    }
+    return (uint32_t)rem;
+}
+int big_bits(const Big& x) {
+    for (int i = 7; i >= 0; --i)
+        if (x.w[i]) return 32 * i + 32 - __builtin_clz(x.w[i]);
+    return 0;
+}
+// bits [lo, lo + 40) of x as an exact double
+double big_digit(const Big& x, int lo) {
+    uint64_t v = 0;
+    for (int b = 39; b >= 0; --b) {
+        int bit = lo + b;
+        uint32_t word = (bit >= 0 && bit < 256) ? x.w[bit / 32] : 0;
+        v = (v << 1) | ((bit >= 0 && bit < 256) ? ((word >> (bit % 32)) & 1u) : 0u);
+    }
+    return (double)v;
+}
+int inv_mod(int a, int m) {  // a^-1 mod m (a, m coprime)
+    int t = 0, nt = 1, r = m, nr = a % m;
+    while (nr) {
+        int q = r / nr;
+        int tmp = t - q * nt; t = nt; nt = tmp;
+        tmp = r - q * nr; r = nr; nr = tmp;
+    }
+    return t < 0 ? t + m : t;
+}
+}  // namespace
+
+bool crt_setup(int nmod, int64_t k, CrtConsts& cc) {
+    if (nmod < 2 || nmod > kMaxModuli) return false;
+    memset(&cc, 0, sizeof(cc));
+    Big P;
+    P.w[0] = 1;
+    for (int l = 0; l < nmod; ++l) big_mul_small(P, (uint32_t)kModuli[l]);
+    const int pbits = big_bits(P);
+    // digit grid: digits of 40 bits, the top digit at bit `top`
+    int nd = (pbits + 8 + 39) / 40;  // X < nmod * P < 2^(pbits + 5)
+    if (nd > kMaxDigits) return false;
+    cc.nmod = nmod;
+    cc.ndig = nd;
+    for (int d = 0; d < nd; ++d) cc.dig_exp[d] = 40 * (nd - 1 - d);
+    for (int l = 0; l < nmod; ++l) {
+        Big Ml = P;
+        big_div_small(Ml, (uint32_t)kModuli[l]);
+        Big tmp = Ml;
+        const uint32_t ml_mod = big_div_small(tmp, (uint32_t)kModuli[l]);
+        cc.m[l] = kModuli[l];
+        cc.y[l] = inv_mod((int)ml_mod, kModuli[l]);
+        cc.inv_m[l] = 1.0 / kModuli[l];
+        for (int d = 0; d < nd; ++d) cc.h[l][d] = big_digit(Ml, cc.dig_exp[d]);
+    }
+    for (int d = 0; d < nd; ++d) cc.p[d] = big_digit(P, cc.dig_exp[d]);
+    // 1/P scaled per digit: q = round(sum_d S_d 2^e_d / P)
+    double p_approx = 0.0;
+    for (int d = 0; d < nd; ++d) p_approx += ldexp(cc.p[d], cc.dig_exp[d]);
+    for (int d = 0; d < nd; ++d) cc.inv_p_dig[d] = ldexp(1.0, cc.dig_exp[d]) / p_approx;
+    // Largest alpha with (2^alpha + sqrt(k)/2)^2 <= P/2 * (1 - 2^-30):
+    // |C'| <= ||a'|| ||b'|| stays clear of P/2 so q = round(X/P) is exact.
+    const double log2p = log2(p_approx);
+    int alpha = (int)floor((log2p - 1.0) / 2.0);
+    const double half_sqrt_k = 0.5 * sqrt((double)(k > 0 ? k : 1));
+    while (alpha > 0) {
+        const double lhs = 2.0 * log2(ldexp(1.0, alpha) + half_sqrt_k);
+        if (lhs <= log2p - 1.0 - 1e-6) break;
+        --alpha;
+    }
+    cc.alpha = alpha;
+    cc.log2p = log2p;
+    return alpha >= 8;
+}
+
+// ---------------------------------------------------------------------------
+// operand statistics: per-row (of op(X)) max |x| and scaled sum of squares
+// element (r, k) of op(X) is x[k + r*ld] when kcontig, x[r + k*ld] otherwise
+// ---------------------------------------------------------------------------
+__global__ void fill_u64_kernel(uint64_t* p, int64_t n, uint64_t v) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+// grid: (ceil(rows / 256), ceil(k / 1024)); one thread per row over a 1024-k chunk (strided
+// rows), or one wave per row chunk (contiguous rows).
+template <bool KCONTIG>
+__global__ __launch_bounds__(256) void row_absmax_kernel(const double* __restrict__ x, int64_t ld, int rows, int k,
+                                                          uint64_t* __restrict__ out_bits) {
+    if constexpr (!KCONTIG) {
+        const int r = blockIdx.x * 256 + threadIdx.x;
+        const int k0 = blockIdx.y * 1024, k1 = min(k, k0 + 1024);
+        if (r >= rows) return;
+        double mx = 0.0;
+        for (int kk = k0; kk < k1; ++kk) mx = fmax(mx, fabs(x[r + (int64_t)kk * ld]));
+        atomicMax((unsigned long long*)&out_bits[r], (unsigned long long)__double_as_longlong(mx));
+    } else {
+        // 4 waves, each handles 64 rows of this block's row range, lanes over k
+        const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+        const int k0 = blockIdx.y * 1024, k1 = min(k, k0 + 1024);
+        for (int rr = wave; rr < 256; rr += 4) {
+            const int r = blockIdx.x * 256 + rr;
+            if (r >= rows) break;
+            double mx = 0.0;
+            for (int kk = k0 + lane; kk < k1; kk += 64) mx = fmax(mx, fabs(x[kk + (int64_t)r * ld]));
+            for (int off = 32; off > 0; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off));
+            if (lane == 0) atomicMax((unsigned long long*)&out_bits[r], (unsigned long long)__double_as_longlong(mx));
+        }
+    }
+}
+
+// sum of (x * 2^-e_r)^2 with e_r the exponent of the row max (no overflow)
+template <bool KCONTIG>
+__global__ __launch_bounds__(256) void row_sumsq_kernel(const double* __restrict__ x, int64_t ld, int rows, int k,
+                                                         const uint64_t* __restrict__ maxbits,
+                                                         double* __restrict__ out) {
+    if constexpr (!KCONTIG) {
+        const int r = blockIdx.x * 256 + threadIdx.x;
+        const int k0 = blockIdx.y * 1024, k1 = min(k, k0 + 1024);
+        if (r >= rows) return;
+        const double mx = __longlong_as_double((long long)maxbits[r]);
+        if (mx == 0.0) return;
+        const int e = ilogb(mx);
+        double s = 0.0;
+        for (int kk = k0; kk < k1; ++kk) {
+            const double v = ldexp(x[r + (int64_t)kk * ld], -e);
+            s = fma(v, v, s);
+        }
+        atomicAdd(&out[r], s);
+    } else {
+        const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+        const int k0 = blockIdx.y * 1024, k1 = min(k, k0 + 1024);
+        for (int rr = wave; rr < 256; rr += 4) {
+            const int r = blockIdx.x * 256 + rr;
+            if (r >= rows) break;
+            const double mx = __longlong_as_double((long long)maxbits[r]);
+            if (mx == 0.0) continue;
+            const int e = ilogb(mx);
+            double s = 0.0;
+            for (int kk = k0 + lane; kk < k1; kk += 64) {
+                const double v = ldexp(x[kk + (int64_t)r * ld], -e);
+                s = fma(v, v, s);
+            }
+            for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+            if (lane == 0) atomicAdd(&out[r], s);
+        }
+    }
+}
+
+// Per-row scaling exponent sigma_r.
+//   CRT:    ||x_r 2^sigma||_2 <= 2^alpha  (from the scaled sum of squares)
+//   slices: max |x_r 2^sigma| < 1
+__global__ void row_exponent_kernel(int rows, const uint64_t* __restrict__ maxbits, const double* __restrict__ sumsq,
+                                    int alpha, int crt, int* __restrict__ sigma) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= rows) return;
+    const double mx = __longlong_as_double((long long)maxbits[r]);
+    if (mx == 0.0 || !isfinite(mx)) {
+        sigma[r] = 0;
+        return;
+    }
+    const int e = ilogb(mx);  // 2^e <= mx < 2^(e+1)
+    if (crt) {
+        // ||x_r|| = 2^e sqrt(sumsq) <= 2^(e + ceil(log2 sqrt(sumsq)))
+        const double nrm = sqrt(sumsq[r]) * (1.0 + 0x1p-30);  // covers the rounding of the sum
+        const int t = (int)ceil(log2(nrm));
+        sigma[r] = alpha - (e + t);
+    } else {
+        sigma[r] = -(e + 1);  // |x| 2^sigma < 1
+    }
+}
+
+// ---------------------------------------------------------------------------
+// plane extraction: op(X) (rows x k, FP64) -> nplanes int8 planes [rows][kp]
+// (K-contiguous, zero padded to kp).  64 x 64 tiles through LDS so both the
+// FP64 reads and the int8 writes are coalesced whatever the transpose.
+// ---------------------------------------------------------------------------
+struct PlaneArgs {
+    const double* x;
+    int64_t ld;
+    int rows, k, kp;
+    const int* sigma;
+    int8_t* out;
+    int64_t plane_stride;  // bytes between planes
+    int nplanes;
+    int crt;               // 1: residues modulo cc.m[l]; 0: 7-bit slices
+    int m[kMaxModuli];
+    double inv_m[kMaxModuli];
+};
+
+__device__ __forceinline__ int sym_mod(double v, int m, double inv_m) {
+    // v is an integer-valued double, |v| < 2^56; returns v mod m in [-m/2, m/2)
+    const double q = rint(v * inv_m);
+    double r = fma(-q, (double)m, v);  // exact: small integer
+    int ri = (int)r;
+    const int half = m >> 1;
+    if (ri >= half + (m & 1)) ri -= m;
+    if (ri < -half) ri += m;
+    return ri;
+}
+
+template <bool KCONTIG>
+__global__ __launch_bounds__(256) void plane_kernel(PlaneArgs p) {
+    __shared__ double tile[64][65];
+    const int r0 = blockIdx.x * 64, k0 = blockIdx.y * 64;
+    const int t = threadIdx.x;
+    // load: coalesced along the contiguous dimension
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+        const int e = t + 256 * i;
+        const int c = e & 63, o = e >> 6;  // c along contiguous dim
+        const int r = KCONTIG ? o : c, kk = KCONTIG ? c : o;
+        const int gr = r0 + r, gk = k0 + kk;
+        double v = 0.0;
+        if (gr < p.rows && gk < p.k) v = KCONTIG ? p.x[gk + (int64_t)gr * p.ld] : p.x[gr + (int64_t)gk * p.ld];
+        tile[r][kk] = v;
+    }
+    __syncthreads();
+    // each thread: one row, 16 consecutive k -> 16 bytes per plane
+    const int r = t >> 2, kq = (t & 3) * 16;
+    const int gr = r0 + r;
+    if (gr >= p.rows || k0 + kq >= p.kp) return;
+    const int sg = p.sigma[gr];
+    int8_t* dst = p.out + (int64_t)gr * p.kp + k0 + kq;
+    if (p.crt) {
+        for (int l = 0; l < p.nplanes; ++l) {
+            int4 packed;
+            int* pw = reinterpret_cast<int*>(&packed);
+#pragma unroll
+            for (int w = 0; w < 4; ++w) {
+                uint32_t word = 0;
+#pragma unroll
+                for (int b = 0; b < 4; ++b) {
+                    const double a = rint(ldexp(tile[r][kq + 4 * w + b], sg));
+                    const int res = sym_mod(a, p.m[l], p.inv_m[l]);
+                    word |= ((uint32_t)(uint8_t)(int8_t)res) << (8 * b);
+                }
+                pw[w] = (int)word;
+            }
+            *reinterpret_cast<int4*>(dst + (int64_t)l * p.plane_stride) = packed;
+        }
+    } else {
+        // slices: s_1 = rint(x 2^6), then 7 more bits per slice (|s_j| <= 64)
+        double rem[16];
+#pragma unroll
+        for (int b = 0; b < 16; ++b) rem[b] = ldexp(tile[r][kq + b], sg);  // |rem| < 1
+        for (int l = 0; l < p.nplanes; ++l) {
+            const int shift = (l == 0) ? 6 : 6 + 7 * l;
+            int4 packed;
+            int* pw = reinterpret_cast<int*>(&packed);
+#pragma unroll
+            for (int w = 0; w < 4; ++w) {
+                uint32_t word = 0;
+#pragma unroll
+                for (int b = 0; b < 4; ++b) {
+                    const double v = rint(ldexp(rem[4 * w + b], shift));
+                    rem[4 * w + b] = fma(-v, ldexp(1.0, -shift), rem[4 * w + b]);  // exact
+                    word |= ((uint32_t)(uint8_t)(int8_t)(int)v) << (8 * b);
+                }
+                pw[w] = (int)word;
+            }
+            *reinterpret_cast<int4*>(dst + (int64_t)l * p.plane_stride) = packed;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// batched INT8 GEMM over K-contiguous planes: D_z = sum_{(a,b) in pairs(z)} A_a B_b^T
+// ---------------------------------------------------------------------------
+template <int BM, int BN, int WM, int WN>
+__global__ __launch_bounds__(64 * WM * WN) void i8_gemm_kernel(I8Args p) {
+    constexpr int NT = 64 * WM * WN;
+    constexpr int BK = 64;           // one MFMA depth per slice
+    constexpr int RS = BK + 16;      // padded LDS row (bytes): conflict-free 16-byte reads
+    constexpr int WTM = BM / WM, WTN = BN / WN;
+    constexpr int FM = WTM / 16, FN = WTN / 16;
+    constexpr int LA = BM * BK / 16 / NT;  // 16-byte loads per thread (A)
+    constexpr int LB = BN * BK / 16 / NT;
+    static_assert(LA >= 1 && LB >= 1, "tile too small for the workgroup");
+
+    __shared__ __attribute__((aligned(16))) int8_t lds_a[2][BM * RS];
+    __shared__ __attribute__((aligned(16))) int8_t lds_b[2][BN * RS];
+
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm0 = (wave % WM) * WTM, wn0 = (wave / WM) * WTN;
+    const int total = p.tiles_m * p.tiles_n;
+    int tm, tn;
+    grouped_tile(xcd_remap(blockIdx.x, total), p.tiles_m, p.tiles_n, 8, tm, tn);
+    const int m0 = tm * BM, n0 = tn * BN;
+    const I8Out& out = p.outs[blockIdx.z];
+
+    v4i acc[FM][FN];
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+        for (int j = 0; j < FN; ++j) acc[i][j] = v4i{0, 0, 0, 0};
+
+    const int kslices = p.kp / BK;
+    const int nsteps = kslices * out.npairs;
+    int4 ra[LA], rb[LB];
+
+    auto load = [&](int s) {
+        const int pr = s / kslices, ks = s % kslices;
+        const int8_t* a = p.a + (int64_t)out.pairs[pr].a * p.a_plane + (int64_t)ks * BK;
+        const int8_t* b = p.b + (int64_t)out.pairs[pr].b * p.b_plane + (int64_t)ks * BK;
+#pragma unroll
+        for (int i = 0; i < LA; ++i) {
+            const int e = tid + NT * i, row = e >> 2, c = (e & 3) * 16;
+            const int gr = m0 + row;
+            ra[i] = (gr < p.m) ? *reinterpret_cast<const int4*>(a + (int64_t)gr * p.kp + c) : int4{0, 0, 0, 0};
+        }
+#pragma unroll
+        for (int i = 0; i < LB; ++i) {
+            const int e = tid + NT * i, row = e >> 2, c = (e & 3) * 16;
+            const int gr = n0 + row;
+            rb[i] = (gr < p.n) ? *reinterpret_cast<const int4*>(b + (int64_t)gr * p.kp + c) : int4{0, 0, 0, 0};
+        }
+    };
+    auto store = [&](int buf) {
+#pragma unroll
+        for (int i = 0; i < LA; ++i) {
+            const int e = tid + NT * i, row = e >> 2, c = (e & 3) * 16;
+            *reinterpret_cast<int4*>(&lds_a[buf][row * RS + c]) = ra[i];
+        }
+#pragma unroll
+        for (int i = 0; i < LB; ++i) {
+            const int e = tid + NT * i, row = e >> 2, c = (e & 3) * 16;
+            *reinterpret_cast<int4*>(&lds_b[buf][row * RS + c]) = rb[i];
+        }
+    };
+
+    if (nsteps > 0) {
+        load(0);
+        store(0);
+    }
+    __syncthreads();
+    const int lr = lane & 15, lk = (lane >> 4) * 16;
+    for (int s = 0; s < nsteps; ++s) {
+        const int cur = s & 1;
+        const bool more = s + 1 < nsteps;
+        if (more) load(s + 1);
+        v4i af[FM], bf[FN];
+#pragma unroll
+        for (int i = 0; i < FM; ++i) af[i] = *reinterpret_cast<const v4i*>(&lds_a[cur][(wm0 + i * 16 + lr) * RS + lk]);
+#pragma unroll
+        for (int j = 0; j < FN; ++j) bf[j] = *reinterpret_cast<const v4i*>(&lds_b[cur][(wn0 + j * 16 + lr) * RS + lk]);
+#pragma unroll
+        for (int i = 0; i < FM; ++i)
+#pragma unroll
+            for (int j = 0; j < FN; ++j) acc[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(af[i], bf[j], acc[i][j], 0, 0, 0);
+        if (more) store(cur ^ 1);
+        __syncthreads();
+    }
+
+    // epilogue: lane holds rows 4*(lane/16)+r of column lane%16 in each block
+    const int lq = (lane >> 4) * 4;
+    if (p.mode == 0) {
+        const int m = out.modulus, y = out.y;
+        const double inv_m = out.inv_m;
+        uint8_t* u = p.u + (int64_t)blockIdx.z * p.u_plane;
+#pragma unroll
+        for (int i = 0; i < FM; ++i)
+#pragma unroll
+            for (int j = 0; j < FN; ++j) {
+                const int col = n0 + wn0 + j * 16 + lr, row = m0 + wm0 + i * 16 + lq;
+                if (col >= p.n || row >= p.m) continue;
+                uint32_t word = 0;
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const double v = (double)y * (double)acc[i][j][r];  // |v| < 2^40, exact
+                    double q = floor(v * inv_m);
+                    double rr = fma(-q, (double)m, v);  // exact small integer
+                    if (rr >= m) rr -= m;
+                    if (rr < 0) rr += m;
+                    word |= ((uint32_t)(int)rr & 0xffu) << (8 * r);
+                }
+                // rows beyond m land in the padded part of the plane (ldu >= m rounded to 4)
+                *reinterpret_cast<uint32_t*>(u + (int64_t)col * p.ldu + row) = word;
+            }
+    } else {
+        const double sc = out.scale;
+#pragma unroll
+        for (int i = 0; i < FM; ++i)
+#pragma unroll
+            for (int j = 0; j < FN; ++j) {
+                const int col = n0 + wn0 + j * 16 + lr, row = m0 + wm0 + i * 16 + lq;
+                if (col >= p.n) continue;
+                double* w = p.w + (int64_t)col * p.ldw + row;
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    if (row + r >= p.m) break;
+                    const double v = (double)acc[i][j][r] * sc;
+                    w[r] = p.accumulate ? w[r] + v : v;
+                }
+            }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// CRT reconstruction + inverse scaling + alpha/beta epilogue
+// ---------------------------------------------------------------------------
+struct CrtArgs {
+    int m, n;
+    const uint8_t* u;
+    int64_t u_plane;
+    int ldu;
+    const int* sig_a;
+    const int* sig_b;
+    double alpha, beta;
+    double* c;
+    int64_t ldc;
+    CrtConsts cc;
+};
+
+__global__ __launch_bounds__(256) void crt_reconstruct_kernel(CrtArgs p) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    const int j = blockIdx.y;
+    if (i >= p.m) return;
+    const int nd = p.cc.ndig, nm = p.cc.nmod;
+    double S[kMaxDigits] = {0.0, 0.0, 0.0, 0.0};
+    const uint8_t* u = p.u + (int64_t)j * p.ldu + i;
+    for (int l = 0; l < nm; ++l) {
+        const double ul = (double)u[(int64_t)l * p.u_plane];
+#pragma unroll
+        for (int d = 0; d < kMaxDigits; ++d)
+            if (d < nd) S[d] = fma(p.cc.h[l][d], ul, S[d]);  // exact: integer < 2^53
+    }
+    double xq = 0.0;
+#pragma unroll
+    for (int d = 0; d < kMaxDigits; ++d)
+        if (d < nd) xq = fma(S[d], p.cc.inv_p_dig[d], xq);
+    const double q = rint(xq);
+    double cp = 0.0;
+#pragma unroll
+    for (int d = 0; d < kMaxDigits; ++d)
+        if (d < nd) cp += ldexp(fma(-q, p.cc.p[d], S[d]), p.cc.dig_exp[d]);  // digit differences are exact
+    const double prod = ldexp(cp, -(p.sig_a[i] + p.sig_b[j]));
+    double* cptr = p.c + (int64_t)j * p.ldc + i;
+    *cptr = epilogue<double>(p.alpha, p.beta, prod, cptr);
+}
+
+// slices: C = alpha * 2^-(e_a + e_b) W + beta C
+__global__ __launch_bounds__(256) void slices_finish_kernel(int m, int n, const double* __restrict__ w, int64_t ldw,
+                                                            const int* __restrict__ sig_a,
+                                                            const int* __restrict__ sig_b, double alpha, double beta,
+                                                            double* __restrict__ c, int64_t ldc) {
+    const int i = blockIdx.x * 256 + threadIdx.x, j = blockIdx.y;
+    if (i >= m) return;
+    const double prod = ldexp(w[(int64_t)j * ldw + i], -(sig_a[i] + sig_b[j]));
+    double* cptr = c + (int64_t)j * ldc + i;
+    *cptr = epilogue<double>(alpha, beta, prod, cptr);
+}
+
+// ---------------------------------------------------------------------------
+// host driver
+// ---------------------------------------------------------------------------
+namespace {
+struct Workspace {
+    char* base = nullptr;
+    size_t used = 0;
+    template <typename T>
+    T* take(size_t count) {
+        used = (used + 255) & ~size_t(255);
+        T* p = reinterpret_cast<T*>(base + used);
+        used += count * sizeof(T);
+        return p;
+    }
+};
+
+hipError_t operand_stats(const double* x, int64_t ld, int rows, int k, bool kcontig, uint64_t* maxbits,
+                         double* sumsq, bool need_sumsq, hipStream_t s) {
+    fill_u64_kernel<<<std::max(1, std::min(1024, (rows + 255) / 256)), 256, 0, s>>>(maxbits, rows, 0ull);
+    if (need_sumsq) fill_u64_kernel<<<std::max(1, std::min(1024, (rows + 255) / 256)), 256, 0, s>>>(
+        reinterpret_cast<uint64_t*>(sumsq), rows, 0ull);
+    dim3 grid((rows + 255) / 256, (k + 1023) / 1024);
+    if (kcontig) row_absmax_kernel<true><<<grid, 256, 0, s>>>(x, ld, rows, k, maxbits);
+    else row_absmax_kernel<false><<<grid, 256, 0, s>>>(x, ld, rows, k, maxbits);
+    if (need_sumsq) {
+        if (kcontig) row_sumsq_kernel<true><<<grid, 256, 0, s>>>(x, ld, rows, k, maxbits, sumsq);
+        else row_sumsq_kernel<false><<<grid, 256, 0, s>>>(x, ld, rows, k, maxbits, sumsq);
+    }
+    return hipGetLastError();
+}
+
+hipError_t planes(const double* x, int64_t ld, int rows, int k, int kp, bool kcontig, const int* sigma,
+                  int8_t* out, int64_t plane_stride, int nplanes, bool crt, const CrtConsts* cc, hipStream_t s) {
+    PlaneArgs pa;
+    memset(&pa, 0, sizeof(pa));
+    pa.x = x; pa.ld = ld; pa.rows = rows; pa.k = k; pa.kp = kp; pa.sigma = sigma;
+    pa.out = out; pa.plane_stride = plane_stride; pa.nplanes = nplanes; pa.crt = crt ? 1 : 0;
+    if (crt)
+        for (int l = 0; l < nplanes; ++l) { pa.m[l] = cc->m[l]; pa.inv_m[l] = cc->inv_m[l]; }
+    dim3 grid((rows + 63) / 64, kp / 64);
+    if (kcontig) plane_kernel<true><<<grid, 256, 0, s>>>(pa);
+    else plane_kernel<false><<<grid, 256, 0, s>>>(pa);
+    return hipGetLastError();
+}
+}  // namespace
+
+hipError_t i8_gemm_launch(const I8Args& args, int nout, hipStream_t s, int tile) {
+    I8Args p = args;
+    switch (tile) {
+        case 1: {
+            p.tiles_m = (p.m + 127) / 128; p.tiles_n = (p.n + 127) / 128;
+            dim3 grid(p.tiles_m * p.tiles_n, 1, nout);
+            i8_gemm_kernel<128, 128, 2, 2><<<grid, 256, 0, s>>>(p);
+            break;
+        }
+        case 2: {
+            p.tiles_m = (p.m + 255) / 256; p.tiles_n = (p.n + 255) / 256;
+            dim3 grid(p.tiles_m * p.tiles_n, 1, nout);
+            i8_gemm_kernel<256, 256, 2, 2><<<grid, 256, 0, s>>>(p);
+            break;
+        }
+        default: {
+            p.tiles_m = (p.m + 255) / 256; p.tiles_n = (p.n + 127) / 128;
+            dim3 grid(p.tiles_m * p.tiles_n, 1, nout);
+            i8_gemm_kernel<256, 128, 2, 2><<<grid, 256, 0, s>>>(p);
+            break;
+        }
+    }
+    return hipGetLastError();
+}
+
+int ozaki_dgemm(bool ta, bool tb, int m, int n, int k, double alpha, const double* a, int64_t lda, const double* b,
+                int64_t ldb, double beta, double* c, int64_t ldc, hipStream_t s, int mode, int mode_param) {
+    if (mode == ZYNUM_GPU_MODE_OZAKI_BF16) return ozaki_bf16_dgemm(ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, s, mode_param);
+    const bool crt = (mode == ZYNUM_GPU_MODE_OZAKI_CRT);
+    const int planes_n = crt ? (mode_param ? mode_param : 14) : (mode_param ? mode_param : 7);
+    if (crt && (planes_n < 8 || planes_n > kMaxModuli)) return ZYNUM_GPU_DECLINE_UNSUPPORTED;
+    if (!crt && (planes_n < 2 || planes_n > 8)) return ZYNUM_GPU_DECLINE_UNSUPPORTED;
+    if ((int64_t)k > (crt ? (1 << 17) : (1 << 18))) return ZYNUM_GPU_DECLINE_UNSUPPORTED;  // INT32 headroom
+    CrtConsts cc;
+    if (crt && !crt_setup(planes_n, k, cc)) return ZYNUM_GPU_DECLINE_UNSUPPORTED;
+
+    const int kp = (k + 63) / 64 * 64;
+    const int ldu = (m + 3) / 4 * 4;
+    const size_t plane_a = (size_t)m * kp, plane_b = (size_t)n * kp;
+    size_t bytes = 0;
+    bytes += 2 * ((size_t)m + (size_t)n) * 8 + 4096;             // max bits + sumsq
+    bytes += ((size_t)m + (size_t)n) * 4 + 1024;                 // sigma
+    bytes += (plane_a + plane_b) * planes_n + 1024;              // planes
+    bytes += crt ? (size_t)ldu * n * planes_n + 256 : (size_t)m * n * 8 + 256;  // u bytes / FP64 level sums
+    bytes += 4096;
+    void* wsp = workspace_acquire(bytes);
+    if (!wsp) return ZYNUM_GPU_DECLINE_ALLOC;
+    Workspace ws{static_cast<char*>(wsp), 0};
+    uint64_t* max_a = ws.take<uint64_t>(m);
+    uint64_t* max_b = ws.take<uint64_t>(n);
+    double* ss_a = ws.take<double>(m);
+    double* ss_b = ws.take<double>(n);
+    int* sig_a = ws.take<int>(m);
+    int* sig_b = ws.take<int>(n);
+    int8_t* pa = ws.take<int8_t>(plane_a * planes_n);
+    int8_t* pb = ws.take<int8_t>(plane_b * planes_n);
+
+    hipError_t e = hipSuccess;
+    // op(A) rows are contiguous in k when transposed; op(B) "rows" (its columns) when not
+    e = operand_stats(a, lda, m, k, ta, max_a, ss_a, crt, s);
+    if (e == hipSuccess) e = operand_stats(b, ldb, n, k, !tb, max_b, ss_b, crt, s);
+    if (e == hipSuccess) {
+        row_exponent_kernel<<<(m + 255) / 256, 256, 0, s>>>(m, max_a, ss_a, cc.alpha, crt, sig_a);
+        row_exponent_kernel<<<(n + 255) / 256, 256, 0, s>>>(n, max_b, ss_b, cc.alpha, crt, sig_b);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = planes(a, lda, m, k, kp, ta, sig_a, pa, plane_a, planes_n, crt, &cc, s);
+    if (e == hipSuccess) e = planes(b, ldb, n, k, kp, !tb, sig_b, pb, plane_b, planes_n, crt, &cc, s);
+
+    I8Args ia;
+    memset(&ia, 0, sizeof(ia));
+    ia.m = m; ia.n = n; ia.kp = kp;
+    ia.a = pa; ia.a_plane = plane_a; ia.b = pb; ia.b_plane = plane_b;
+    if (crt && e == hipSuccess) {
+        uint8_t* u = ws.take<uint8_t>((size_t)ldu * n * planes_n);
+        ia.mode = 0;
+        ia.u = u; ia.u_plane = (int64_t)ldu * n; ia.ldu = ldu;
+        for (int l = 0; l < planes_n; ++l) {
+            ia.outs[l].npairs = 1;
+            ia.outs[l].pairs[0] = {l, l};
+            ia.outs[l].modulus = cc.m[l];
+            ia.outs[l].y = cc.y[l];
+            ia.outs[l].inv_m = cc.inv_m[l];
+        }
+        e = i8_gemm_launch(ia, planes_n, s, 0);
+        if (e == hipSuccess) {
+            CrtArgs ca;
+            ca.m = m; ca.n = n; ca.u = u; ca.u_plane = ia.u_plane; ca.ldu = ldu;
+            ca.sig_a = sig_a; ca.sig_b = sig_b; ca.alpha = alpha; ca.beta = beta; ca.c = c; ca.ldc = ldc;
+            ca.cc = cc;
+            dim3 grid((m + 255) / 256, n);
+            crt_reconstruct_kernel<<<grid, 256, 0, s>>>(ca);
+            e = hipGetLastError();
+        }
+    } else if (e == hipSuccess) {
+        double* w = ws.take<double>((size_t)m * n);
+        ia.mode = 1;
+        ia.w = w; ia.ldw = m;
+        // level t (2..s+1): pairs (i, t - i) of 1-based slices, scale 2^-(12 + 7 (t - 2))
+        for (int t = 2; t <= planes_n + 1 && e == hipSuccess; ++t) {
+            I8Args lv = ia;
+            int np = 0;
+            for (int i = 1; i < t; ++i) {
+                const int j = t - i;
+                if (i > planes_n || j > planes_n) continue;
+                lv.outs[0].pairs[np++] = {i - 1, j - 1};
+            }
+            lv.outs[0].npairs = np;
+            lv.outs[0].scale = ldexp(1.0, -(12 + 7 * (t - 2)));
+            lv.accumulate = (t == 2) ? 0 : 1;
+            e = i8_gemm_launch(lv, 1, s, 0);
+        }
+        if (e == hipSuccess) {
+            dim3 grid((m + 255) / 256, n);
+            slices_finish_kernel<<<grid, 256, 0, s>>>(m, n, w, m, sig_a, sig_b, alpha, beta, c, ldc);
+            e = hipGetLastError();
+        }
+    }
+    int st = status_from_hip(e);
+    if (st == ZYNUM_GPU_OK) st = status_from_hip(hipStreamSynchronize(s));  // workspace reuse is host-ordered
+    workspace_release();
+    return st;
+}
+
+}  // namespace zg
+
+namespace zg {
+int ozaki_bf16_dgemm(bool, bool, int, int, int, double, const double*, int64_t, const double*, int64_t, double,
+                     double*, int64_t, hipStream_t, int) {
     return ZYNUM_GPU_DECLINE_UNSUPPORTED;
 }
 }  // namespace zg

@@ -1,0 +1,69 @@
+// Split-precision (Ozaki) DGEMM: shared types of the INT8 GEMM core and the
+// CRT reconstruction.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace zg {
+
+constexpr int kMaxModuli = 20;
+constexpr int kMaxDigits = 4;
+
+// Chinese-remainder constants for the first `nmod` moduli.  M_l = P / m_l and
+// P are split into 40-bit digits: M_l = sum_d h[l][d] 2^dig_exp[d].
+struct CrtConsts {
+    int nmod, ndig, alpha;
+    int m[kMaxModuli];
+    int y[kMaxModuli];  // (P / m_l)^-1 mod m_l
+    double inv_m[kMaxModuli];
+    double h[kMaxModuli][kMaxDigits];
+    double p[kMaxDigits];
+    double inv_p_dig[kMaxDigits];  // 2^dig_exp[d] / P
+    int dig_exp[kMaxDigits];
+    double log2p;
+};
+
+// Fills `cc` for `nmod` moduli and inner dimension k; false if unsupported.
+bool crt_setup(int nmod, int64_t k, CrtConsts& cc);
+
+struct I8Pair {
+    int a, b;  // plane indices of op(A) and op(B)
+};
+
+struct I8Out {
+    int npairs;
+    I8Pair pairs[8];
+    double scale;   // FP64-accumulate mode: weight of this output
+    int modulus;    // residue mode: m_l
+    int y;          // residue mode: (P / m_l)^-1 mod m_l
+    double inv_m;
+};
+
+// Batched INT8 GEMM over K-contiguous planes [rows][kp] (kp a multiple of 64).
+// Output z (grid.z) sums the products of its plane pairs in INT32, then
+//   mode 0: stores u = (y * acc) mod m as one byte at u + z*u_plane + col*ldu + row
+//   mode 1: w[col*ldw + row] (+)= acc * scale
+struct I8Args {
+    int m, n, kp;
+    const int8_t* a;
+    int64_t a_plane;
+    const int8_t* b;
+    int64_t b_plane;
+    int tiles_m, tiles_n;
+    int mode;
+    uint8_t* u;
+    int64_t u_plane;
+    int ldu;
+    double* w;
+    int64_t ldw;
+    int accumulate;
+    I8Out outs[kMaxModuli];
+};
+
+hipError_t i8_gemm_launch(const I8Args& args, int nout, hipStream_t s, int tile);
+
+int ozaki_bf16_dgemm(bool ta, bool tb, int m, int n, int k, double alpha, const double* a, int64_t lda,
+                     const double* b, int64_t ldb, double beta, double* c, int64_t ldc, hipStream_t s, int slices);
+
+}  // namespace zg
