@@ -113,8 +113,9 @@ def test_crt_error_bound_random(torch_cuda, nmod):
 
 def test_wide_range_no_worse_than_plain_fp64(torch_cuda):
     """On the wide-range workload shape (a = u 2^e, e in [-16, 16]) the 14-residue
-    CRT form's max error against double-double is no larger than the plain FP64
-    MFMA DGEMM's; 7-slice Ozaki I stays within 4x of it."""
+    CRT form's and the 8-slice Ozaki I form's max errors against double-double
+    are no larger than the plain FP64 MFMA DGEMM's (7 slices are not enough on
+    this input: max-exponent row scaling leaves ~6 bits of range unused)."""
     from zynum_amd import synth
 
     n = 1024
@@ -124,13 +125,13 @@ def test_wide_range_no_worse_than_plain_fp64(torch_cuda):
     hi, lo = dd_ref(torch_cuda, "N", "N", n, n, n, 1.0, a, n, b, n, 0.0, c, n)
     plain = run(torch_cuda, "N", "N", n, n, n, 1.0, a, n, b, n, 0.0, c, n, 0, 0)
     crt = run(torch_cuda, "N", "N", n, n, n, 1.0, a, n, b, n, 0.0, c, n, 3, 14)
-    sl7 = run(torch_cuda, "N", "N", n, n, n, 1.0, a, n, b, n, 0.0, c, n, 1, 7)
+    sl8 = run(torch_cuda, "N", "N", n, n, n, 1.0, a, n, b, n, 0.0, c, n, 1, 8)
     e_plain = np.max(np.abs((plain - hi) - lo))
     e_crt = np.max(np.abs((crt - hi) - lo))
-    e_sl7 = np.max(np.abs((sl7 - hi) - lo))
-    print(f"max err vs dd: plain {e_plain:.3e}  crt14 {e_crt:.3e}  slices7 {e_sl7:.3e}")
+    e_sl8 = np.max(np.abs((sl8 - hi) - lo))
+    print(f"max err vs dd: plain {e_plain:.3e}  crt14 {e_crt:.3e}  slices8 {e_sl8:.3e}")
     assert e_crt <= e_plain
-    assert e_sl7 <= 4 * e_plain
+    assert e_sl8 <= e_plain
 
 
 def test_slices_error_decreases_with_slices(torch_cuda):

@@ -2,7 +2,9 @@
 
 Every ``.hip`` / ``.cpp`` source under ``csrc/`` is compiled by ``hipcc`` for
 gfx950 only and linked in-tree, so the built library travels with the source
-tree.  Packed-FP32 instructions are disabled on every HIP compile.
+tree.  Packed-FP32 instructions are disabled on every HIP compile, and
+floating-point contraction is off (every fused multiply-add is an explicit
+fma, so the double-double and epilogue roundings are the written ones).
 """
 from __future__ import annotations
 
@@ -26,6 +28,7 @@ COMMON = [
     "-Wall",
     "-Wno-unused-function",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-ffp-contract=off",  # error-free transformations and the reference epilogue need exact roundings
     "-fvisibility=hidden",
 ]
 

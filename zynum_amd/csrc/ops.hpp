@@ -23,7 +23,7 @@ int syrk_op(char uplo, bool trans, int n, int k, T alpha, const T* a, int64_t ld
 // Device-pointer implementations (no staging), stream ordered.
 int ozaki_dgemm(bool ta, bool tb, int m, int n, int k, double alpha, const double* a, int64_t lda,
                 const double* b, int64_t ldb, double beta, double* c, int64_t ldc, hipStream_t s, int mode,
-                int mode_param);
+                int mode_param, int tile = -1);
 int dd_dgemm(bool ta, bool tb, int m, int n, int k, double alpha, const double* a, int64_t lda, const double* b,
              int64_t ldb, double beta, double* c, int64_t ldc, hipStream_t s);
 template <typename T>

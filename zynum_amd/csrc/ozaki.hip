@@ -147,76 +147,65 @@ __global__ void fill_u64_kernel(uint64_t* p, int64_t n, uint64_t v) {
         p[i] = v;
 }
 
-// grid: (ceil(rows / 256), ceil(k / 1024)); one thread per row over a 1024-k chunk (strided
-// rows), or one wave per row chunk (contiguous rows).
-template <bool KCONTIG>
-__global__ __launch_bounds__(256) void row_absmax_kernel(const double* __restrict__ x, int64_t ld, int rows, int k,
-                                                          uint64_t* __restrict__ out_bits) {
-    if constexpr (!KCONTIG) {
-        const int r = blockIdx.x * 256 + threadIdx.x;
-        const int k0 = blockIdx.y * 1024, k1 = min(k, k0 + 1024);
-        if (r >= rows) return;
-        double mx = 0.0;
-        for (int kk = k0; kk < k1; ++kk) mx = fmax(mx, fabs(x[r + (int64_t)kk * ld]));
-        atomicMax((unsigned long long*)&out_bits[r], (unsigned long long)__double_as_longlong(mx));
-    } else {
-        // 4 waves, each handles 64 rows of this block's row range, lanes over k
-        const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-        const int k0 = blockIdx.y * 1024, k1 = min(k, k0 + 1024);
-        for (int rr = wave; rr < 256; rr += 4) {
-            const int r = blockIdx.x * 256 + rr;
-            if (r >= rows) break;
-            double mx = 0.0;
-            for (int kk = k0 + lane; kk < k1; kk += 64) mx = fmax(mx, fabs(x[kk + (int64_t)r * ld]));
-            for (int off = 32; off > 0; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off));
-            if (lane == 0) atomicMax((unsigned long long*)&out_bits[r], (unsigned long long)__double_as_longlong(mx));
-        }
-    }
-}
+// One pass over op(X): per-row max |x| (atomicMax on the bit pattern, which
+// orders like the value for non-negative doubles) and raw sum of squares.
+// Strided rows: one thread per row over a 256-deep k chunk.  Contiguous rows:
+// one wave per row, lanes over k.
+constexpr int kStatChunk = 256;
 
-// sum of (x * 2^-e_r)^2 with e_r the exponent of the row max (no overflow)
 template <bool KCONTIG>
-__global__ __launch_bounds__(256) void row_sumsq_kernel(const double* __restrict__ x, int64_t ld, int rows, int k,
-                                                         const uint64_t* __restrict__ maxbits,
-                                                         double* __restrict__ out) {
+__global__ __launch_bounds__(256) void row_stats_kernel(const double* __restrict__ x, int64_t ld, int rows, int k,
+                                                         uint64_t* __restrict__ maxbits, double* __restrict__ sumsq) {
+    const int k0 = blockIdx.y * kStatChunk, k1 = min(k, k0 + kStatChunk);
     if constexpr (!KCONTIG) {
         const int r = blockIdx.x * 256 + threadIdx.x;
-        const int k0 = blockIdx.y * 1024, k1 = min(k, k0 + 1024);
         if (r >= rows) return;
-        const double mx = __longlong_as_double((long long)maxbits[r]);
-        if (mx == 0.0) return;
-        const int e = ilogb(mx);
-        double s = 0.0;
-        for (int kk = k0; kk < k1; ++kk) {
-            const double v = ldexp(x[r + (int64_t)kk * ld], -e);
-            s = fma(v, v, s);
+        double mx = 0.0, ss = 0.0;
+        const double* px = x + r + (int64_t)k0 * ld;
+        int kk = k0;
+        for (; kk + 4 <= k1; kk += 4, px += 4 * ld) {
+            const double v0 = px[0], v1 = px[ld], v2 = px[2 * ld], v3 = px[3 * ld];
+            mx = fmax(fmax(mx, fabs(v0)), fmax(fabs(v1), fmax(fabs(v2), fabs(v3))));
+            ss = fma(v0, v0, ss); ss = fma(v1, v1, ss); ss = fma(v2, v2, ss); ss = fma(v3, v3, ss);
         }
-        atomicAdd(&out[r], s);
+        for (; kk < k1; ++kk, px += ld) {
+            const double v = *px;
+            mx = fmax(mx, fabs(v));
+            ss = fma(v, v, ss);
+        }
+        atomicMax((unsigned long long*)&maxbits[r], (unsigned long long)__double_as_longlong(mx));
+        atomicAdd(&sumsq[r], ss);
     } else {
         const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-        const int k0 = blockIdx.y * 1024, k1 = min(k, k0 + 1024);
-        for (int rr = wave; rr < 256; rr += 4) {
-            const int r = blockIdx.x * 256 + rr;
+        for (int rr = wave; rr < 64; rr += 4) {
+            const int r = blockIdx.x * 64 + rr;
             if (r >= rows) break;
-            const double mx = __longlong_as_double((long long)maxbits[r]);
-            if (mx == 0.0) continue;
-            const int e = ilogb(mx);
-            double s = 0.0;
+            double mx = 0.0, ss = 0.0;
             for (int kk = k0 + lane; kk < k1; kk += 64) {
-                const double v = ldexp(x[kk + (int64_t)r * ld], -e);
-                s = fma(v, v, s);
+                const double v = x[kk + (int64_t)r * ld];
+                mx = fmax(mx, fabs(v));
+                ss = fma(v, v, ss);
             }
-            for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
-            if (lane == 0) atomicAdd(&out[r], s);
+            for (int off = 32; off > 0; off >>= 1) {
+                mx = fmax(mx, __shfl_xor(mx, off));
+                ss += __shfl_xor(ss, off);
+            }
+            if (lane == 0) {
+                atomicMax((unsigned long long*)&maxbits[r], (unsigned long long)__double_as_longlong(mx));
+                atomicAdd(&sumsq[r], ss);
+            }
         }
     }
 }
 
 // Per-row scaling exponent sigma_r.
-//   CRT:    ||x_r 2^sigma||_2 <= 2^alpha  (from the scaled sum of squares)
+//   CRT:    ||x_r 2^sigma||_2 <= 2^alpha.  The norm comes from the raw sum of
+//           squares when that is trustworthy (finite, and the max square is a
+//           normal number so no significant term underflowed); otherwise from
+//           the bound sqrt(k) max |x|.
 //   slices: max |x_r 2^sigma| < 1
-__global__ void row_exponent_kernel(int rows, const uint64_t* __restrict__ maxbits, const double* __restrict__ sumsq,
-                                    int alpha, int crt, int* __restrict__ sigma) {
+__global__ void row_exponent_kernel(int rows, int k, const uint64_t* __restrict__ maxbits,
+                                    const double* __restrict__ sumsq, int alpha, int crt, int* __restrict__ sigma) {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= rows) return;
     const double mx = __longlong_as_double((long long)maxbits[r]);
@@ -224,14 +213,17 @@ __global__ void row_exponent_kernel(int rows, const uint64_t* __restrict__ maxbi
         sigma[r] = 0;
         return;
     }
-    const int e = ilogb(mx);  // 2^e <= mx < 2^(e+1)
     if (crt) {
-        // ||x_r|| = 2^e sqrt(sumsq) <= 2^(e + ceil(log2 sqrt(sumsq)))
-        const double nrm = sqrt(sumsq[r]) * (1.0 + 0x1p-30);  // covers the rounding of the sum
-        const int t = (int)ceil(log2(nrm));
-        sigma[r] = alpha - (e + t);
+        const double m2 = mx * mx, ss = sumsq[r];
+        const bool ok = isfinite(ss) && isfinite(m2) && m2 >= 0x1p-1000 && ss >= m2;
+        // relative rounding of a k-term sum is below k 2^-53 <= 2^-36 for k <= 2^17
+        const double nrm = ok ? sqrt(ss) * (1.0 + 0x1p-30) : sqrt((double)k) * mx * (1.0 + 0x1p-30);
+        int ex;
+        const double fr = frexp(nrm, &ex);  // nrm = fr 2^ex, fr in [0.5, 1)
+        (void)fr;
+        sigma[r] = alpha - ex;  // ||x|| 2^sigma < 2^ex 2^(alpha - ex) = 2^alpha
     } else {
-        sigma[r] = -(e + 1);  // |x| 2^sigma < 1
+        sigma[r] = -(ilogb(mx) + 1);  // |x| 2^sigma < 1
     }
 }
 
@@ -332,19 +324,22 @@ __global__ __launch_bounds__(256) void plane_kernel(PlaneArgs p) {
 // ---------------------------------------------------------------------------
 // batched INT8 GEMM over K-contiguous planes: D_z = sum_{(a,b) in pairs(z)} A_a B_b^T
 // ---------------------------------------------------------------------------
-template <int BM, int BN, int WM, int WN>
+template <int BM, int BN, int BK, int WM, int WN>
 __global__ __launch_bounds__(64 * WM * WN) void i8_gemm_kernel(I8Args p) {
     constexpr int NT = 64 * WM * WN;
-    constexpr int BK = 64;           // one MFMA depth per slice
-    constexpr int RS = BK + 16;      // padded LDS row (bytes): conflict-free 16-byte reads
+    constexpr int RS = BK + 16;      // padded LDS row (bytes): 16-byte reads of 16 rows hit distinct banks
+    constexpr int CPR = BK / 16;     // 16-byte chunks per row
     constexpr int WTM = BM / WM, WTN = BN / WN;
     constexpr int FM = WTM / 16, FN = WTN / 16;
-    constexpr int LA = BM * BK / 16 / NT;  // 16-byte loads per thread (A)
-    constexpr int LB = BN * BK / 16 / NT;
-    static_assert(LA >= 1 && LB >= 1, "tile too small for the workgroup");
+    constexpr int LA = BM * CPR / NT;  // 16-byte loads per thread (A)
+    constexpr int LB = BN * CPR / NT;
+    static_assert(LA >= 1 && LB >= 1 && BM * CPR % NT == 0 && BN * CPR % NT == 0, "tile/workgroup mismatch");
+    static_assert(BK % 64 == 0, "BK is a multiple of the MFMA depth");
 
-    __shared__ __attribute__((aligned(16))) int8_t lds_a[2][BM * RS];
-    __shared__ __attribute__((aligned(16))) int8_t lds_b[2][BN * RS];
+    extern __shared__ __attribute__((aligned(16))) int8_t lds[];
+    // stage buf: A rows at lds + buf*(BM+BN)*RS, B rows right after them
+    auto stage_a = [&](int buf) { return lds + buf * (BM + BN) * RS; };
+    auto stage_b = [&](int buf) { return lds + buf * (BM + BN) * RS + BM * RS; };
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm0 = (wave % WM) * WTM, wn0 = (wave / WM) * WTN;
@@ -370,13 +365,13 @@ __global__ __launch_bounds__(64 * WM * WN) void i8_gemm_kernel(I8Args p) {
         const int8_t* b = p.b + (int64_t)out.pairs[pr].b * p.b_plane + (int64_t)ks * BK;
 #pragma unroll
         for (int i = 0; i < LA; ++i) {
-            const int e = tid + NT * i, row = e >> 2, c = (e & 3) * 16;
+            const int e = tid + NT * i, row = e / CPR, c = (e % CPR) * 16;
             const int gr = m0 + row;
             ra[i] = (gr < p.m) ? *reinterpret_cast<const int4*>(a + (int64_t)gr * p.kp + c) : int4{0, 0, 0, 0};
         }
 #pragma unroll
         for (int i = 0; i < LB; ++i) {
-            const int e = tid + NT * i, row = e >> 2, c = (e & 3) * 16;
+            const int e = tid + NT * i, row = e / CPR, c = (e % CPR) * 16;
             const int gr = n0 + row;
             rb[i] = (gr < p.n) ? *reinterpret_cast<const int4*>(b + (int64_t)gr * p.kp + c) : int4{0, 0, 0, 0};
         }
@@ -384,13 +379,13 @@ __global__ __launch_bounds__(64 * WM * WN) void i8_gemm_kernel(I8Args p) {
     auto store = [&](int buf) {
 #pragma unroll
         for (int i = 0; i < LA; ++i) {
-            const int e = tid + NT * i, row = e >> 2, c = (e & 3) * 16;
-            *reinterpret_cast<int4*>(&lds_a[buf][row * RS + c]) = ra[i];
+            const int e = tid + NT * i, row = e / CPR, c = (e % CPR) * 16;
+            *reinterpret_cast<int4*>(stage_a(buf) + row * RS + c) = ra[i];
         }
 #pragma unroll
         for (int i = 0; i < LB; ++i) {
-            const int e = tid + NT * i, row = e >> 2, c = (e & 3) * 16;
-            *reinterpret_cast<int4*>(&lds_b[buf][row * RS + c]) = rb[i];
+            const int e = tid + NT * i, row = e / CPR, c = (e % CPR) * 16;
+            *reinterpret_cast<int4*>(stage_b(buf) + row * RS + c) = rb[i];
         }
     };
 
@@ -404,15 +399,21 @@ __global__ __launch_bounds__(64 * WM * WN) void i8_gemm_kernel(I8Args p) {
         const int cur = s & 1;
         const bool more = s + 1 < nsteps;
         if (more) load(s + 1);
-        v4i af[FM], bf[FN];
+        const int8_t* la = stage_a(cur);
+        const int8_t* lb = stage_b(cur);
 #pragma unroll
-        for (int i = 0; i < FM; ++i) af[i] = *reinterpret_cast<const v4i*>(&lds_a[cur][(wm0 + i * 16 + lr) * RS + lk]);
+        for (int kk = 0; kk < BK; kk += 64) {
+            v4i af[FM], bf[FN];
 #pragma unroll
-        for (int j = 0; j < FN; ++j) bf[j] = *reinterpret_cast<const v4i*>(&lds_b[cur][(wn0 + j * 16 + lr) * RS + lk]);
+            for (int i = 0; i < FM; ++i) af[i] = *reinterpret_cast<const v4i*>(&la[(wm0 + i * 16 + lr) * RS + kk + lk]);
 #pragma unroll
-        for (int i = 0; i < FM; ++i)
+            for (int j = 0; j < FN; ++j) bf[j] = *reinterpret_cast<const v4i*>(&lb[(wn0 + j * 16 + lr) * RS + kk + lk]);
 #pragma unroll
-            for (int j = 0; j < FN; ++j) acc[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(af[i], bf[j], acc[i][j], 0, 0, 0);
+            for (int i = 0; i < FM; ++i)
+#pragma unroll
+                for (int j = 0; j < FN; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(af[i], bf[j], acc[i][j], 0, 0, 0);
+        }
         if (more) store(cur ^ 1);
         __syncthreads();
     }
@@ -477,31 +478,42 @@ struct CrtArgs {
     CrtConsts cc;
 };
 
+// one thread per 4 consecutive rows of one column (one 32-bit load per plane)
 __global__ __launch_bounds__(256) void crt_reconstruct_kernel(CrtArgs p) {
-    const int i = blockIdx.x * 256 + threadIdx.x;
+    const int i0 = (blockIdx.x * 256 + threadIdx.x) * 4;
     const int j = blockIdx.y;
-    if (i >= p.m) return;
+    if (i0 >= p.m) return;
     const int nd = p.cc.ndig, nm = p.cc.nmod;
-    double S[kMaxDigits] = {0.0, 0.0, 0.0, 0.0};
-    const uint8_t* u = p.u + (int64_t)j * p.ldu + i;
+    double S[4][kMaxDigits] = {};
+    const uint8_t* u = p.u + (int64_t)j * p.ldu + i0;
     for (int l = 0; l < nm; ++l) {
-        const double ul = (double)u[(int64_t)l * p.u_plane];
+        const uint32_t w = *reinterpret_cast<const uint32_t*>(u + (int64_t)l * p.u_plane);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const double ul = (double)((w >> (8 * r)) & 0xffu);
+#pragma unroll
+            for (int d = 0; d < kMaxDigits; ++d)
+                if (d < nd) S[r][d] = fma(p.cc.h[l][d], ul, S[r][d]);  // exact: integer < 2^53
+        }
+    }
+    const int sb = p.sig_b[j];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int i = i0 + r;
+        if (i >= p.m) break;
+        double xq = 0.0;
 #pragma unroll
         for (int d = 0; d < kMaxDigits; ++d)
-            if (d < nd) S[d] = fma(p.cc.h[l][d], ul, S[d]);  // exact: integer < 2^53
+            if (d < nd) xq = fma(S[r][d], p.cc.inv_p_dig[d], xq);
+        const double q = rint(xq);
+        double cp = 0.0;
+#pragma unroll
+        for (int d = 0; d < kMaxDigits; ++d)
+            if (d < nd) cp += ldexp(fma(-q, p.cc.p[d], S[r][d]), p.cc.dig_exp[d]);  // exact digit differences
+        const double prod = ldexp(cp, -(p.sig_a[i] + sb));
+        double* cptr = p.c + (int64_t)j * p.ldc + i;
+        *cptr = epilogue<double>(p.alpha, p.beta, prod, cptr);
     }
-    double xq = 0.0;
-#pragma unroll
-    for (int d = 0; d < kMaxDigits; ++d)
-        if (d < nd) xq = fma(S[d], p.cc.inv_p_dig[d], xq);
-    const double q = rint(xq);
-    double cp = 0.0;
-#pragma unroll
-    for (int d = 0; d < kMaxDigits; ++d)
-        if (d < nd) cp += ldexp(fma(-q, p.cc.p[d], S[d]), p.cc.dig_exp[d]);  // digit differences are exact
-    const double prod = ldexp(cp, -(p.sig_a[i] + p.sig_b[j]));
-    double* cptr = p.c + (int64_t)j * p.ldc + i;
-    *cptr = epilogue<double>(p.alpha, p.beta, prod, cptr);
 }
 
 // slices: C = alpha * 2^-(e_a + e_b) W + beta C
@@ -533,16 +545,16 @@ struct Workspace {
 };
 
 hipError_t operand_stats(const double* x, int64_t ld, int rows, int k, bool kcontig, uint64_t* maxbits,
-                         double* sumsq, bool need_sumsq, hipStream_t s) {
-    fill_u64_kernel<<<std::max(1, std::min(1024, (rows + 255) / 256)), 256, 0, s>>>(maxbits, rows, 0ull);
-    if (need_sumsq) fill_u64_kernel<<<std::max(1, std::min(1024, (rows + 255) / 256)), 256, 0, s>>>(
-        reinterpret_cast<uint64_t*>(sumsq), rows, 0ull);
-    dim3 grid((rows + 255) / 256, (k + 1023) / 1024);
-    if (kcontig) row_absmax_kernel<true><<<grid, 256, 0, s>>>(x, ld, rows, k, maxbits);
-    else row_absmax_kernel<false><<<grid, 256, 0, s>>>(x, ld, rows, k, maxbits);
-    if (need_sumsq) {
-        if (kcontig) row_sumsq_kernel<true><<<grid, 256, 0, s>>>(x, ld, rows, k, maxbits, sumsq);
-        else row_sumsq_kernel<false><<<grid, 256, 0, s>>>(x, ld, rows, k, maxbits, sumsq);
+                         double* sumsq, hipStream_t s) {
+    const int fb = std::max(1, std::min(1024, (rows + 255) / 256));
+    fill_u64_kernel<<<fb, 256, 0, s>>>(maxbits, rows, 0ull);
+    fill_u64_kernel<<<fb, 256, 0, s>>>(reinterpret_cast<uint64_t*>(sumsq), rows, 0ull);
+    if (kcontig) {
+        dim3 grid((rows + 63) / 64, (k + kStatChunk - 1) / kStatChunk);
+        row_stats_kernel<true><<<grid, 256, 0, s>>>(x, ld, rows, k, maxbits, sumsq);
+    } else {
+        dim3 grid((rows + 255) / 256, (k + kStatChunk - 1) / kStatChunk);
+        row_stats_kernel<false><<<grid, 256, 0, s>>>(x, ld, rows, k, maxbits, sumsq);
     }
     return hipGetLastError();
 }
@@ -562,33 +574,40 @@ hipError_t planes(const double* x, int64_t ld, int rows, int k, int kp, bool kco
 }
 }  // namespace
 
-hipError_t i8_gemm_launch(const I8Args& args, int nout, hipStream_t s, int tile) {
-    I8Args p = args;
-    switch (tile) {
-        case 1: {
-            p.tiles_m = (p.m + 127) / 128; p.tiles_n = (p.n + 127) / 128;
-            dim3 grid(p.tiles_m * p.tiles_n, 1, nout);
-            i8_gemm_kernel<128, 128, 2, 2><<<grid, 256, 0, s>>>(p);
-            break;
-        }
-        case 2: {
-            p.tiles_m = (p.m + 255) / 256; p.tiles_n = (p.n + 255) / 256;
-            dim3 grid(p.tiles_m * p.tiles_n, 1, nout);
-            i8_gemm_kernel<256, 256, 2, 2><<<grid, 256, 0, s>>>(p);
-            break;
-        }
-        default: {
-            p.tiles_m = (p.m + 255) / 256; p.tiles_n = (p.n + 127) / 128;
-            dim3 grid(p.tiles_m * p.tiles_n, 1, nout);
-            i8_gemm_kernel<256, 128, 2, 2><<<grid, 256, 0, s>>>(p);
-            break;
-        }
-    }
+template <int BM, int BN, int BK, int WM, int WN>
+static hipError_t i8_launch_cfg(I8Args p, int nout, hipStream_t s) {
+    p.tiles_m = (p.m + BM - 1) / BM;
+    p.tiles_n = (p.n + BN - 1) / BN;
+    if (p.kp % BK != 0) return hipErrorInvalidValue;
+    const size_t lds = 2 * (size_t)(BM + BN) * (BK + 16);
+    auto kern = i8_gemm_kernel<BM, BN, BK, WM, WN>;
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)lds);
+    if (e != hipSuccess) return e;
+    dim3 grid(p.tiles_m * p.tiles_n, 1, nout);
+    kern<<<grid, 64 * WM * WN, lds, s>>>(p);
     return hipGetLastError();
 }
 
+// INT8 tile configurations (BM x BN x BK, wave grid); kp must be a multiple of BK.
+hipError_t i8_gemm_launch(const I8Args& args, int nout, hipStream_t s, int tile) {
+    switch (tile) {
+        case 0: return i8_launch_cfg<256, 256, 128, 2, 2>(args, nout, s);
+        case 1: return i8_launch_cfg<256, 128, 64, 2, 2>(args, nout, s);
+        case 2: return i8_launch_cfg<128, 128, 128, 2, 2>(args, nout, s);
+        case 3: return i8_launch_cfg<256, 256, 64, 2, 2>(args, nout, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+int i8_pick_tile(int m, int n, int kp) {
+    const long long big = (long long)((m + 255) / 256) * ((n + 255) / 256);
+    if (kp % 128 == 0 && big >= 64) return 0;
+    return 1;
+}
+
 int ozaki_dgemm(bool ta, bool tb, int m, int n, int k, double alpha, const double* a, int64_t lda, const double* b,
-                int64_t ldb, double beta, double* c, int64_t ldc, hipStream_t s, int mode, int mode_param) {
+                int64_t ldb, double beta, double* c, int64_t ldc, hipStream_t s, int mode, int mode_param, int tile) {
     if (mode == ZYNUM_GPU_MODE_OZAKI_BF16) return ozaki_bf16_dgemm(ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, s, mode_param);
     const bool crt = (mode == ZYNUM_GPU_MODE_OZAKI_CRT);
     const int planes_n = crt ? (mode_param ? mode_param : 14) : (mode_param ? mode_param : 7);
@@ -598,7 +617,7 @@ int ozaki_dgemm(bool ta, bool tb, int m, int n, int k, double alpha, const doubl
     CrtConsts cc;
     if (crt && !crt_setup(planes_n, k, cc)) return ZYNUM_GPU_DECLINE_UNSUPPORTED;
 
-    const int kp = (k + 63) / 64 * 64;
+    const int kp = (k + 127) / 128 * 128;
     const int ldu = (m + 3) / 4 * 4;
     const size_t plane_a = (size_t)m * kp, plane_b = (size_t)n * kp;
     size_t bytes = 0;
@@ -621,11 +640,11 @@ int ozaki_dgemm(bool ta, bool tb, int m, int n, int k, double alpha, const doubl
 
     hipError_t e = hipSuccess;
     // op(A) rows are contiguous in k when transposed; op(B) "rows" (its columns) when not
-    e = operand_stats(a, lda, m, k, ta, max_a, ss_a, crt, s);
-    if (e == hipSuccess) e = operand_stats(b, ldb, n, k, !tb, max_b, ss_b, crt, s);
+    e = operand_stats(a, lda, m, k, ta, max_a, ss_a, s);
+    if (e == hipSuccess) e = operand_stats(b, ldb, n, k, !tb, max_b, ss_b, s);
     if (e == hipSuccess) {
-        row_exponent_kernel<<<(m + 255) / 256, 256, 0, s>>>(m, max_a, ss_a, cc.alpha, crt, sig_a);
-        row_exponent_kernel<<<(n + 255) / 256, 256, 0, s>>>(n, max_b, ss_b, cc.alpha, crt, sig_b);
+        row_exponent_kernel<<<(m + 255) / 256, 256, 0, s>>>(m, k, max_a, ss_a, cc.alpha, crt, sig_a);
+        row_exponent_kernel<<<(n + 255) / 256, 256, 0, s>>>(n, k, max_b, ss_b, cc.alpha, crt, sig_b);
         e = hipGetLastError();
     }
     if (e == hipSuccess) e = planes(a, lda, m, k, kp, ta, sig_a, pa, plane_a, planes_n, crt, &cc, s);
@@ -646,13 +665,13 @@ int ozaki_dgemm(bool ta, bool tb, int m, int n, int k, double alpha, const doubl
             ia.outs[l].y = cc.y[l];
             ia.outs[l].inv_m = cc.inv_m[l];
         }
-        e = i8_gemm_launch(ia, planes_n, s, 0);
+        e = i8_gemm_launch(ia, planes_n, s, tile >= 0 ? tile : i8_pick_tile(m, n, kp));
         if (e == hipSuccess) {
             CrtArgs ca;
             ca.m = m; ca.n = n; ca.u = u; ca.u_plane = ia.u_plane; ca.ldu = ldu;
             ca.sig_a = sig_a; ca.sig_b = sig_b; ca.alpha = alpha; ca.beta = beta; ca.c = c; ca.ldc = ldc;
             ca.cc = cc;
-            dim3 grid((m + 255) / 256, n);
+            dim3 grid((m + 1023) / 1024, n);
             crt_reconstruct_kernel<<<grid, 256, 0, s>>>(ca);
             e = hipGetLastError();
         }
@@ -672,7 +691,7 @@ int ozaki_dgemm(bool ta, bool tb, int m, int n, int k, double alpha, const doubl
             lv.outs[0].npairs = np;
             lv.outs[0].scale = ldexp(1.0, -(12 + 7 * (t - 2)));
             lv.accumulate = (t == 2) ? 0 : 1;
-            e = i8_gemm_launch(lv, 1, s, 0);
+            e = i8_gemm_launch(lv, 1, s, tile >= 0 ? tile : i8_pick_tile(m, n, kp));
         }
         if (e == hipSuccess) {
             dim3 grid((m + 255) / 256, n);

@@ -8,7 +8,7 @@
 namespace zg {
 
 constexpr int kMaxModuli = 20;
-constexpr int kMaxDigits = 4;
+constexpr int kMaxDigits = 5;
 
 // Chinese-remainder constants for the first `nmod` moduli.  M_l = P / m_l and
 // P are split into 40-bit digits: M_l = sum_d h[l][d] 2^dig_exp[d].
@@ -62,6 +62,7 @@ struct I8Args {
 };
 
 hipError_t i8_gemm_launch(const I8Args& args, int nout, hipStream_t s, int tile);
+int i8_pick_tile(int m, int n, int kp);
 
 int ozaki_bf16_dgemm(bool ta, bool tb, int m, int n, int k, double alpha, const double* a, int64_t lda,
                      const double* b, int64_t ldb, double beta, double* c, int64_t ldc, hipStream_t s, int slices);
