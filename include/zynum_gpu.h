@@ -159,12 +159,15 @@ typedef struct zynum_gpu_gemm_problem {
 } zynum_gpu_gemm_problem; /* 72 bytes */
 
 /* Opaque grouped-GEMM plan: the problem list is validated, bucketed by size,
-   assigned a tile shape per bucket by the autotuner table, and uploaded once.
+   assigned a tile shape per bucket by the autotuner table (tile = 0), or a
+   forced tile configuration (tile = i + 1: 32x32, 64x64), expanded
+   into output tiles in longest-first order and uploaded once.
    zynum_gpu_grouped_run issues one persistent launch per batch; it allocates
-   nothing and may be captured into a HIP graph. */
+   nothing and may be captured into a HIP graph.  Problem pointers are
+   captured by the plan: rerun it after rewriting the same buffers. */
 typedef struct zynum_gpu_grouped_plan zynum_gpu_grouped_plan;
 
-int zynum_gpu_grouped_plan_create(int scalar, const zynum_gpu_gemm_problem *problems, int count,
+int zynum_gpu_grouped_plan_create(int scalar, const zynum_gpu_gemm_problem *problems, int count, int tile,
                                   zynum_gpu_grouped_plan **plan_out);
 int zynum_gpu_grouped_run(zynum_gpu_grouped_plan *plan, void *stream);
 /* Capture the persistent launch into a HIP graph owned by the plan and replay it. */

@@ -84,7 +84,7 @@ SIGNATURES = {
     "zynum_blas_shutdown": (None, []),
     "zynum_blas_shutdown_": (None, []),
     "zynum_gpu_gemm": (C.c_int, [C.POINTER(GemmRequest)]),
-    "zynum_gpu_grouped_plan_create": (C.c_int, [C.c_int, C.POINTER(GemmProblem), C.c_int,
+    "zynum_gpu_grouped_plan_create": (C.c_int, [C.c_int, C.POINTER(GemmProblem), C.c_int, C.c_int,
                                                 C.POINTER(C.c_void_p)]),
     "zynum_gpu_grouped_run": (C.c_int, [C.c_void_p, C.c_void_p]),
     "zynum_gpu_grouped_graph_run": (C.c_int, [C.c_void_p, C.c_void_p]),

@@ -340,7 +340,7 @@ int zynum_gpu_gemm(const zynum_gpu_gemm_request* r) {
     const char tb = r->transb == ZYNUM_GPU_NO_TRANS ? 'N' : r->transb <= ZYNUM_GPU_CONJ_TRANS ? 'T' : 'X';
     if (gemm_error(ta, tb, r->m, r->n, r->k, r->lda, r->ldb, r->ldc) != 0) return ZYNUM_GPU_DECLINE_ARGUMENT;
     if (r->alpha_im != 0 || r->beta_im != 0) return ZYNUM_GPU_DECLINE_UNSUPPORTED;
-    if (r->tile > (r->mode == ZYNUM_GPU_MODE_PLAIN ? zg::kTileConfigs : 4)) return ZYNUM_GPU_DECLINE_UNSUPPORTED;
+    if (r->tile > (r->mode == ZYNUM_GPU_MODE_PLAIN ? zg::kTileConfigs : 6)) return ZYNUM_GPU_DECLINE_UNSUPPORTED;
     hipStream_t s = r->stream ? reinterpret_cast<hipStream_t>(r->stream) : zg::blas_stream();
     if (r->scalar == ZYNUM_GPU_F64) {
         return zg::gemm_op<double>(ta == 'T', tb == 'T', r->m, r->n, r->k, scalar_from_bits<double>(r->alpha_re),

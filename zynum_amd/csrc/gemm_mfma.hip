@@ -20,196 +20,19 @@
 //     tiles in grouped (super-row) order, keeping shared A/B panels in its L2.
 #include "common.hpp"
 #include "gemm.hpp"
+#include "gemm_tile.hpp"
 
 namespace zg {
-
-template <typename T, int N>
-using vec_t = T __attribute__((ext_vector_type(N)));
-
-template <typename T>
-struct GemmArgs {
-    int m, n, k;
-    int64_t lda, ldb, ldc;
-    T alpha, beta;
-    const T* __restrict__ a;
-    const T* __restrict__ b;
-    T* __restrict__ c;
-    int tiles_m, tiles_n, group_m;
-};
-
-// Moves one R x BK slice of an operand (R = tile rows of op(X)) from global
-// memory to registers and from registers to a k-major LDS panel [BK][R + PAD].
-// KCONTIG: element (r, kk) is at x[(r0 + r) * ldx + k0 + kk] (contiguous in k),
-// otherwise at x[(k0 + kk) * ldx + r0 + r] (contiguous in r).
-template <typename T, int R, int BK, bool KCONTIG, int VEC, int NT, int PAD>
-struct Stage {
-    static constexpr int E = R * BK / NT;  // elements per thread
-    static constexpr int NV = E / VEC;     // vector loads per thread
-    static_assert(E % VEC == 0, "tile not divisible");
-    static constexpr int PER = KCONTIG ? BK / VEC : R / VEC;  // vectors along the contiguous dim
-
-    T v[E];
-
-    __device__ __forceinline__ void load(const T* __restrict__ x, int64_t ldx, int r0, int k0, int rlim,
-                                         int klim, bool full, int tid) {
-#pragma unroll
-        for (int i = 0; i < NV; ++i) {
-            const int idx = tid + NT * i;
-            const int c = (idx % PER) * VEC;  // position along the contiguous dim
-            const int o = idx / PER;          // position along the other dim
-            const int r = KCONTIG ? o : c;
-            const int kk = KCONTIG ? c : o;
-            const int64_t off = KCONTIG ? (int64_t)(r0 + r) * ldx + (k0 + kk) : (int64_t)(k0 + kk) * ldx + (r0 + r);
-            if (full) {
-                if constexpr (VEC == 1) {
-                    v[i] = x[off];
-                } else {
-                    vec_t<T, VEC> t = *reinterpret_cast<const vec_t<T, VEC>*>(x + off);
-#pragma unroll
-                    for (int e = 0; e < VEC; ++e) v[i * VEC + e] = t[e];
-                }
-            } else {
-#pragma unroll
-                for (int e = 0; e < VEC; ++e) {
-                    const int rr = r0 + r + (KCONTIG ? 0 : e);
-                    const int kv = k0 + kk + (KCONTIG ? e : 0);
-                    v[i * VEC + e] = (rr < rlim && kv < klim) ? x[off + e] : T(0);
-                }
-            }
-        }
-    }
-
-    __device__ __forceinline__ void store(T* __restrict__ lds, int tid) const {
-#pragma unroll
-        for (int i = 0; i < NV; ++i) {
-            const int idx = tid + NT * i;
-            const int c = (idx % PER) * VEC;
-            const int o = idx / PER;
-            if constexpr (KCONTIG) {
-                // r = o, kk = c..c+VEC-1: transpose into the k-major panel
-#pragma unroll
-                for (int e = 0; e < VEC; ++e) lds[(c + e) * (R + PAD) + o] = v[i * VEC + e];
-            } else {
-                // kk = o, r = c..c+VEC-1: contiguous in the panel row
-                if constexpr (VEC == 1) {
-                    lds[o * (R + PAD) + c] = v[i];
-                } else {
-                    vec_t<T, VEC> t;
-#pragma unroll
-                    for (int e = 0; e < VEC; ++e) t[e] = v[i * VEC + e];
-                    *reinterpret_cast<vec_t<T, VEC>*>(lds + o * (R + PAD) + c) = t;
-                }
-            }
-        }
-    }
-};
-
-// Tile configuration: BM x BN tile per workgroup, BK-deep K slices, WM x WN
-// wave grid (each wave owns (BM/WM) x (BN/WN)), OCC = waves per SIMD the
-// register budget is sized for (0 = compiler's choice).
-template <int BM_, int BN_, int BK_, int WM_, int WN_, int OCC_>
-struct TileCfg {
-    static constexpr int BM = BM_, BN = BN_, BK = BK_, WM = WM_, WN = WN_, OCC = OCC_;
-    static constexpr int NT = 64 * WM_ * WN_;
-};
 
 template <typename T, class CF, bool TA, bool TB, int VEC>
 __global__ __launch_bounds__(CF::NT) __attribute__((amdgpu_waves_per_eu(CF::OCC > 0 ? CF::OCC : 1)))
 void gemm_mfma_kernel(GemmArgs<T> p) {
-    constexpr int BM = CF::BM, BN = CF::BN, BK = CF::BK, NT = CF::NT;
-    constexpr int PAD = 16 / sizeof(T);
-    constexpr int WTM = BM / CF::WM, WTN = BN / CF::WN;
-    constexpr int FM = WTM / 16, FN = WTN / 16;
-    static_assert(FM >= 1 && FN >= 1, "wave tile smaller than one MFMA block");
-    using M = Mfma<T>;
-    using Acc = typename M::acc_t;
-
-    __shared__ __attribute__((aligned(16))) T lds_a[2][BK * (BM + PAD)];
-    __shared__ __attribute__((aligned(16))) T lds_b[2][BK * (BN + PAD)];
-
-    const int tid = threadIdx.x;
-    const int lane = tid & 63;
-    const int wave = tid >> 6;
-    const int wm0 = (wave % CF::WM) * WTM;
-    const int wn0 = (wave / CF::WM) * WTN;
-
+    __shared__ __attribute__((aligned(16))) T lds_a[tile_lds_a<T, CF>()];
+    __shared__ __attribute__((aligned(16))) T lds_b[tile_lds_b<T, CF>()];
     const int total = p.tiles_m * p.tiles_n;
     int tm, tn;
     grouped_tile(xcd_remap(blockIdx.x, total), p.tiles_m, p.tiles_n, p.group_m, tm, tn);
-    const int m0 = tm * BM, n0 = tn * BN;
-
-    // op(A) tile rows are M (A: M-contiguous unless TA); op(B) rows are N
-    // (B^T: N-contiguous when TB, otherwise K-contiguous).
-    Stage<T, BM, BK, TA, VEC, NT, PAD> sa;
-    Stage<T, BN, BK, !TB, VEC, NT, PAD> sb;
-
-    Acc acc[FM][FN];
-#pragma unroll
-    for (int i = 0; i < FM; ++i)
-#pragma unroll
-        for (int j = 0; j < FN; ++j) acc[i][j] = M::zero();
-
-    const int nk = (p.k + BK - 1) / BK;
-    const bool mn_full = (m0 + BM <= p.m) && (n0 + BN <= p.n);
-
-    sa.load(p.a, p.lda, m0, 0, p.m, p.k, mn_full && BK <= p.k, tid);
-    sb.load(p.b, p.ldb, n0, 0, p.n, p.k, mn_full && BK <= p.k, tid);
-    sa.store(lds_a[0], tid);
-    sb.store(lds_b[0], tid);
-    __syncthreads();
-
-    const int lr = lane & 15;  // column within a 16x16 block (B, D)
-    const int lk = lane >> 4;  // k within a 4-deep step
-    const int ar = M::a_row(lr);  // A row fed by this lane
-
-    for (int t = 0; t < nk; ++t) {
-        const int cur = t & 1;
-        const bool more = (t + 1) < nk;
-        if (more) {
-            const int k0 = (t + 1) * BK;
-            const bool full = mn_full && (k0 + BK <= p.k);
-            sa.load(p.a, p.lda, m0, k0, p.m, p.k, full, tid);
-            sb.load(p.b, p.ldb, n0, k0, p.n, p.k, full, tid);
-        }
-        const T* __restrict__ la = lds_a[cur];
-        const T* __restrict__ lb = lds_b[cur];
-#pragma unroll
-        for (int kk = 0; kk < BK; kk += 4) {
-            T af[FM], bf[FN];
-#pragma unroll
-            for (int i = 0; i < FM; ++i) af[i] = la[(kk + lk) * (BM + PAD) + wm0 + i * 16 + ar];
-#pragma unroll
-            for (int j = 0; j < FN; ++j) bf[j] = lb[(kk + lk) * (BN + PAD) + wn0 + j * 16 + lr];
-#pragma unroll
-            for (int i = 0; i < FM; ++i)
-#pragma unroll
-                for (int j = 0; j < FN; ++j) acc[i][j] = M::mma(af[i], bf[j], acc[i][j]);
-        }
-        if (more) {
-            sa.store(lds_a[cur ^ 1], tid);
-            sb.store(lds_b[cur ^ 1], tid);
-        }
-        __syncthreads();
-    }
-
-    // Epilogue: lane holds rows 4*(lane/16)+r, column lane%16 of each block,
-    // i.e. 4 consecutive rows of one C column.
-    const T alpha = p.alpha, beta = p.beta;
-#pragma unroll
-    for (int i = 0; i < FM; ++i) {
-#pragma unroll
-        for (int j = 0; j < FN; ++j) {
-            const int col = n0 + wn0 + j * 16 + lr;
-            const int row = m0 + wm0 + i * 16 + 4 * lk;
-            if (col < p.n) {
-                T* cp = p.c + (int64_t)col * p.ldc + row;
-#pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    if (row + r < p.m) cp[r] = epilogue<T>(alpha, beta, acc[i][j][r], cp + r);
-                }
-            }
-        }
-    }
+    gemm_tile<T, CF, TA, TB, VEC>(p, tm * CF::BM, tn * CF::BN, lds_a, lds_b);
 }
 
 template <typename T>

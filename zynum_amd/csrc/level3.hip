@@ -1,22 +1,336 @@
-// DTRSM / DSYRK on the GEMM core (filled in below the GEMM core).
+// DTRSM and DSYRK on the GEMM core.
+//
+// TRSM (triangular.zig:155-249 solves one column at a time with trsv and
+// streams the whole triangle per right-hand side) is recast recursively, as
+// in the reference's blocked template (structured_blocked.zig:315-374):
+// split the triangle in two halves, solve the first, update the second with
+// one GEMM (B2 -= op(A)21 X1, plain FP64 MFMA or split precision), recurse.
+// Leaves of at most 64 rows are solved in FP64 from LDS: the 64x64 triangle is
+// staged once per workgroup and each wave64 solves one vector (a column of B
+// for side L, a row for side R) with one lane per unknown, in the reference's
+// per-step order x_k /= a_kk; x_i = fma(a_ik, -x_k, x_i)
+// (matrix_vector/triangular.zig:375-385).
+//
+// SYRK (symmetric.zig:180-279, one strided dot per element) runs as one launch
+// over the lower (or upper) triangle of output tiles with the GEMM tile routine;
+// diagonal tiles write only their stored triangle, so the other triangle is
+// never touched (structured_blocked.zig:158-201 commits the same way).
+#include <math.h>
+
+#include <algorithm>
+
 #include "common.hpp"
 #include "gemm.hpp"
+#include "gemm_tile.hpp"
 #include "ops.hpp"
 #include "runtime.hpp"
 
 namespace zg {
 
+// ---------------------------------------------------------------------------
+// B <- alpha * B (a plain multiply, as scaleDenseColumns, triangular.zig:52-59)
+// ---------------------------------------------------------------------------
 template <typename T>
-int trsm_device(char, char, bool, char, int, int, T, const T*, int64_t, T*, int64_t, hipStream_t, int, int) {
-    return ZYNUM_GPU_DECLINE_UNSUPPORTED;
+__global__ void mul_kernel(int m, int n, T alpha, T* __restrict__ b, int64_t ldb) {
+    const int64_t total = (int64_t)m * n;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t j = e / m, i = e % m;
+        b[j * ldb + i] = alpha * b[j * ldb + i];
+    }
 }
+
+// ---------------------------------------------------------------------------
+// leaf solve: nb <= 64 unknowns per vector
+// ---------------------------------------------------------------------------
+constexpr int kLeaf = 64;
+
 template <typename T>
-int syrk_device(char, bool, int, int, T, const T*, int64_t, T, T*, int64_t, hipStream_t, int, int) {
-    return ZYNUM_GPU_DECLINE_UNSUPPORTED;
+struct LeafArgs {
+    const T* a;
+    int64_t lda;
+    T* b;
+    int64_t ldb;
+    int left;      // side L: vectors are columns of B, unknowns are rows i0..i0+nb
+    int lower_eff; // op(A) is lower triangular
+    int trans, unit;
+    int i0, nb;    // unknowns [i0, i0 + nb) along the triangle
+    int nvec;      // number of vectors (n for side L, m for side R)
+};
+
+template <typename T>
+__device__ __forceinline__ T op_a(const LeafArgs<T>& p, int r, int c) {
+    return p.trans ? p.a[(int64_t)r * p.lda + c] : p.a[(int64_t)c * p.lda + r];
 }
-template int trsm_device<double>(char, char, bool, char, int, int, double, const double*, int64_t, double*, int64_t, hipStream_t, int, int);
-template int trsm_device<float>(char, char, bool, char, int, int, float, const float*, int64_t, float*, int64_t, hipStream_t, int, int);
-template int syrk_device<double>(char, bool, int, int, double, const double*, int64_t, double, double*, int64_t, hipStream_t, int, int);
-template int syrk_device<float>(char, bool, int, int, float, const float*, int64_t, float, float*, int64_t, hipStream_t, int, int);
+
+// grid.x: groups of 64 vectors; 256 threads = 4 waves, 16 vectors per wave
+template <typename T>
+__global__ __launch_bounds__(256) void trsm_leaf_kernel(LeafArgs<T> p) {
+    __shared__ T M[kLeaf][kLeaf + 1];  // M[i][k]: coefficient of unknown k in equation i
+    __shared__ T V[kLeaf][kLeaf + 1];  // V[v][i]: vector v, unknown i
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int v0 = blockIdx.x * kLeaf;
+    const int nb = p.nb;
+    // Equation matrix: side L solves op(A) x = b  (M = op(A) block),
+    // side R solves x op(A) = b, i.e. op(A)^T x^T = b^T (M = op(A)^T block).
+    // M is lower iff (side L and op(A) lower) or (side R and op(A) upper).
+    const bool m_lower = p.left ? p.lower_eff : !p.lower_eff;
+    for (int e = tid; e < kLeaf * kLeaf; e += 256) {
+        const int i = e % kLeaf, k = e / kLeaf;
+        T v = T(0);
+        if (i < nb && k < nb) {
+            if (i == k) {
+                v = p.unit ? T(1) : (p.left ? op_a(p, p.i0 + i, p.i0 + k) : op_a(p, p.i0 + k, p.i0 + i));
+            } else if (m_lower ? (i > k) : (i < k)) {
+                v = p.left ? op_a(p, p.i0 + i, p.i0 + k) : op_a(p, p.i0 + k, p.i0 + i);
+            }
+        }
+        M[i][k] = v;
+    }
+    // vectors: side L, vector v = column v0 + v of B, unknown i = row i0 + i;
+    // side R, vector v = row v0 + v, unknown i = column i0 + i
+    for (int e = tid; e < kLeaf * kLeaf; e += 256) {
+        int v, i;
+        if (p.left) { i = e % kLeaf; v = e / kLeaf; } else { v = e % kLeaf; i = e / kLeaf; }
+        T x = T(0);
+        if (i < nb && v0 + v < p.nvec)
+            x = p.left ? p.b[(int64_t)(v0 + v) * p.ldb + p.i0 + i] : p.b[(int64_t)(p.i0 + i) * p.ldb + v0 + v];
+        V[v][i] = x;
+    }
+    __syncthreads();
+    for (int v = wave; v < kLeaf; v += 4) {
+        if (v0 + v >= p.nvec) break;
+        T x = V[v][lane];
+        if (m_lower) {
+            for (int k = 0; k < nb; ++k) {
+                if (lane == k && !p.unit) x = x / M[k][k];
+                const T xk = __shfl(x, k);
+                if (lane > k && lane < nb) x = fma_t(M[lane][k], -xk, x);
+            }
+        } else {
+            for (int k = nb - 1; k >= 0; --k) {
+                if (lane == k && !p.unit) x = x / M[k][k];
+                const T xk = __shfl(x, k);
+                if (lane < k) x = fma_t(M[lane][k], -xk, x);
+            }
+        }
+        V[v][lane] = x;
+    }
+    __syncthreads();
+    for (int e = tid; e < kLeaf * kLeaf; e += 256) {
+        int v, i;
+        if (p.left) { i = e % kLeaf; v = e / kLeaf; } else { v = e % kLeaf; i = e / kLeaf; }
+        if (i < nb && v0 + v < p.nvec) {
+            if (p.left) p.b[(int64_t)(v0 + v) * p.ldb + p.i0 + i] = V[v][i];
+            else p.b[(int64_t)(p.i0 + i) * p.ldb + v0 + v] = V[v][i];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// recursion
+// ---------------------------------------------------------------------------
+namespace {
+template <typename T>
+struct TrsmCtx {
+    bool left, lower_eff, trans, unit;
+    int m, n;
+    const T* a;
+    int64_t lda;
+    T* b;
+    int64_t ldb;
+    hipStream_t s;
+    int mode, mode_param;
+};
+
+// op(A)[r0.., c0..] as a GEMM operand (pointer, transpose flag)
+template <typename T>
+const T* sub_opa(const TrsmCtx<T>& c, int r0, int c0) {
+    return c.trans ? c.a + r0 * c.lda + c0 : c.a + c0 * c.lda + r0;
+}
+
+template <typename T>
+int gemm_update(const TrsmCtx<T>& c, bool ta, bool tb, int m, int n, int k, const T* a, int64_t lda, const T* b,
+                int64_t ldb, T* cc, int64_t ldc) {
+    if (m <= 0 || n <= 0 || k <= 0) return ZYNUM_GPU_OK;
+    if constexpr (sizeof(T) == 8) {
+        if (c.mode != ZYNUM_GPU_MODE_PLAIN)
+            return ozaki_dgemm(ta, tb, m, n, k, -1.0, a, lda, b, ldb, 1.0, cc, ldc, c.s, c.mode, c.mode_param);
+    }
+    GemmDesc<T> d;
+    d.ta = ta; d.tb = tb; d.m = m; d.n = n; d.k = k;
+    d.lda = lda; d.ldb = ldb; d.ldc = ldc;
+    d.alpha = T(-1); d.beta = T(1);
+    d.a = a; d.b = b; d.c = cc;
+    return status_from_hip(gemm_launch<T>(d, c.s));
+}
+
+template <typename T>
+int leaf(const TrsmCtx<T>& c, int i0, int nb) {
+    LeafArgs<T> p;
+    p.a = c.a; p.lda = c.lda; p.b = c.b; p.ldb = c.ldb;
+    p.left = c.left; p.lower_eff = c.lower_eff; p.trans = c.trans; p.unit = c.unit;
+    p.i0 = i0; p.nb = nb;
+    p.nvec = c.left ? c.n : c.m;
+    const int grid = (p.nvec + kLeaf - 1) / kLeaf;
+    trsm_leaf_kernel<T><<<grid, 256, 0, c.s>>>(p);
+    return status_from_hip(hipGetLastError());
+}
+
+// solves the unknowns [i0, i0 + len) (rows of X for side L, columns for side R)
+template <typename T>
+int solve(const TrsmCtx<T>& c, int i0, int len) {
+    if (len <= kLeaf) return leaf(c, i0, len);
+    int h = (len / 2 + kLeaf - 1) / kLeaf * kLeaf;
+    if (h >= len) h = len - kLeaf;
+    const int i1 = i0 + h, len2 = len - h;
+    // forward when the equations are lower triangular: side L with op(A) lower,
+    // side R with op(A) upper
+    const bool forward = c.left ? c.lower_eff : !c.lower_eff;
+    int st;
+    if (c.left) {
+        if (forward) {
+            if ((st = solve(c, i0, h))) return st;
+            // B[i1:, :] -= op(A)[i1:, i0:i1] X[i0:i1, :]
+            st = gemm_update(c, c.trans, false, len2, c.n, h, sub_opa(c, i1, i0), c.lda, c.b + i0, c.ldb, c.b + i1,
+                             c.ldb);
+            if (st) return st;
+            return solve(c, i1, len2);
+        }
+        if ((st = solve(c, i1, len2))) return st;
+        // B[i0:i1, :] -= op(A)[i0:i1, i1:] X[i1:, :]
+        st = gemm_update(c, c.trans, false, h, c.n, len2, sub_opa(c, i0, i1), c.lda, c.b + i1, c.ldb, c.b + i0,
+                         c.ldb);
+        if (st) return st;
+        return solve(c, i0, h);
+    }
+    if (forward) {
+        if ((st = solve(c, i0, h))) return st;
+        // B[:, i1:] -= X[:, i0:i1] op(A)[i0:i1, i1:]
+        st = gemm_update(c, false, c.trans, c.m, len2, h, c.b + (int64_t)i0 * c.ldb, c.ldb, sub_opa(c, i0, i1),
+                         c.lda, c.b + (int64_t)i1 * c.ldb, c.ldb);
+        if (st) return st;
+        return solve(c, i1, len2);
+    }
+    if ((st = solve(c, i1, len2))) return st;
+    // B[:, i0:i1] -= X[:, i1:] op(A)[i1:, i0:i1]
+    st = gemm_update(c, false, c.trans, c.m, h, len2, c.b + (int64_t)i1 * c.ldb, c.ldb, sub_opa(c, i1, i0), c.lda,
+                     c.b + (int64_t)i0 * c.ldb, c.ldb);
+    if (st) return st;
+    return solve(c, i0, h);
+}
+}  // namespace
+
+template <typename T>
+int trsm_device(char side, char uplo, bool trans, char diag, int m, int n, T alpha, const T* a, int64_t lda, T* b,
+                int64_t ldb, hipStream_t s, int mode, int mode_param) {
+    if (m <= 0 || n <= 0) return ZYNUM_GPU_OK;
+    if (alpha != T(1)) {
+        const int64_t total = (int64_t)m * n;
+        const int blocks = (int)std::min<int64_t>(65536, (total + 255) / 256);
+        mul_kernel<T><<<blocks, 256, 0, s>>>(m, n, alpha, b, ldb);
+        const int st = status_from_hip(hipGetLastError());
+        if (st) return st;
+    }
+    TrsmCtx<T> c;
+    c.left = (side == 'L');
+    c.lower_eff = (uplo == 'L') != trans;
+    c.trans = trans;
+    c.unit = (diag == 'U');
+    c.m = m; c.n = n; c.a = a; c.lda = lda; c.b = b; c.ldb = ldb; c.s = s;
+    c.mode = mode; c.mode_param = mode_param;
+    return solve(c, 0, c.left ? m : n);
+}
+
+// ---------------------------------------------------------------------------
+// SYRK
+// ---------------------------------------------------------------------------
+using SyrkCfg = TileCfg<128, 128, 16, 4, 2, 2>;
+
+template <typename T, bool TA, int TRI>
+__global__ __launch_bounds__(SyrkCfg::NT) __attribute__((amdgpu_waves_per_eu(2)))
+void syrk_kernel(GemmArgs<T> p, int tiles) {
+    __shared__ __attribute__((aligned(16))) T lds_a[tile_lds_a<T, SyrkCfg>()];
+    __shared__ __attribute__((aligned(16))) T lds_b[tile_lds_b<T, SyrkCfg>()];
+    // linear index -> (ti, tj) with ti >= tj (lower) or ti <= tj (upper)
+    const int t = blockIdx.x;
+    int hi = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
+    while ((hi + 1) * (hi + 2) / 2 <= t) ++hi;
+    while (hi * (hi + 1) / 2 > t) --hi;
+    const int lo = t - hi * (hi + 1) / 2;
+    const int ti = (TRI == 1) ? hi : lo, tj = (TRI == 1) ? lo : hi;
+    (void)tiles;
+    // C = alpha * op(A) op(A)^T + beta * C: the B operand is A with the other transpose
+    gemm_tile<T, SyrkCfg, TA, !TA, 1, TRI>(p, ti * SyrkCfg::BM, tj * SyrkCfg::BN, lds_a, lds_b);
+}
+
+// stored triangle of C <- W + beta * C (W = alpha * op(A) op(A)^T computed in
+// full), or C <- beta * C when w is null (k == 0 or alpha == 0, exact 0 for
+// beta == 0 as symmetric.zig:196)
+template <typename T>
+__global__ void syrk_commit_kernel(int n, int lower, const T* __restrict__ w, int64_t ldw, T beta, T* __restrict__ c,
+                                   int64_t ldc) {
+    const int i = blockIdx.x * 256 + threadIdx.x, j = blockIdx.y;
+    if (i >= n || (lower ? i < j : i > j)) return;
+    T* cp = c + (int64_t)j * ldc + i;
+    const T s = (beta == T(0)) ? T(0) : beta * *cp;
+    *cp = (w != nullptr) ? w[(int64_t)j * ldw + i] + s : s;
+}
+
+template <typename T>
+int syrk_device(char uplo, bool trans, int n, int k, T alpha, const T* a, int64_t lda, T beta, T* c, int64_t ldc,
+                hipStream_t s, int mode, int mode_param) {
+    if (n <= 0) return ZYNUM_GPU_OK;
+    const bool lower = (uplo == 'L');
+    if (k <= 0 || alpha == T(0)) {
+        // C = beta * C on the stored triangle only
+        dim3 grid((n + 255) / 256, n);
+        syrk_commit_kernel<T><<<grid, 256, 0, s>>>(n, lower, nullptr, 0, beta, c, ldc);
+        return status_from_hip(hipGetLastError());
+    }
+    if (sizeof(T) == 8 && mode != ZYNUM_GPU_MODE_PLAIN) {
+        if constexpr (sizeof(T) == 8) {
+            T* w = nullptr;
+            if (hipMalloc(&w, (size_t)n * n * sizeof(T)) != hipSuccess) {
+                (void)hipGetLastError();
+                return ZYNUM_GPU_DECLINE_ALLOC;
+            }
+            int st = ozaki_dgemm(trans, !trans, n, n, k, alpha, a, lda, a, lda, 0.0, w, n, s, mode, mode_param);
+            if (st == ZYNUM_GPU_OK) {
+                dim3 grid((n + 255) / 256, n);
+                syrk_commit_kernel<T><<<grid, 256, 0, s>>>(n, lower, w, n, beta, c, ldc);
+                st = status_from_hip(hipGetLastError());
+                if (st == ZYNUM_GPU_OK) st = status_from_hip(hipStreamSynchronize(s));
+            }
+            (void)hipFree(w);
+            return st;
+        }
+    }
+    GemmArgs<T> p;
+    p.m = n; p.n = n; p.k = k;
+    p.lda = lda; p.ldb = lda; p.ldc = ldc;
+    p.alpha = alpha; p.beta = beta;
+    p.a = a; p.b = a; p.c = c;
+    const int tn = (n + SyrkCfg::BM - 1) / SyrkCfg::BM;
+    p.tiles_m = p.tiles_n = tn;
+    p.group_m = 1;
+    const int tiles = tn * (tn + 1) / 2;
+    if (lower) {
+        if (trans) syrk_kernel<T, true, 1><<<tiles, SyrkCfg::NT, 0, s>>>(p, tiles);
+        else syrk_kernel<T, false, 1><<<tiles, SyrkCfg::NT, 0, s>>>(p, tiles);
+    } else {
+        if (trans) syrk_kernel<T, true, 2><<<tiles, SyrkCfg::NT, 0, s>>>(p, tiles);
+        else syrk_kernel<T, false, 2><<<tiles, SyrkCfg::NT, 0, s>>>(p, tiles);
+    }
+    return status_from_hip(hipGetLastError());
+}
+
+template int trsm_device<double>(char, char, bool, char, int, int, double, const double*, int64_t, double*, int64_t,
+                                 hipStream_t, int, int);
+template int trsm_device<float>(char, char, bool, char, int, int, float, const float*, int64_t, float*, int64_t,
+                                hipStream_t, int, int);
+template int syrk_device<double>(char, bool, int, int, double, const double*, int64_t, double, double*, int64_t,
+                                 hipStream_t, int, int);
+template int syrk_device<float>(char, bool, int, int, float, const float*, int64_t, float, float*, int64_t,
+                                hipStream_t, int, int);
 
 }  // namespace zg

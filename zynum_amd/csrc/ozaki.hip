@@ -246,9 +246,16 @@ struct PlaneArgs {
 };
 
 __device__ __forceinline__ int sym_mod(double v, int m, double inv_m) {
-    // v is an integer-valued double, |v| < 2^56; returns v mod m in [-m/2, m/2)
-    const double q = rint(v * inv_m);
-    double r = fma(-q, (double)m, v);  // exact: small integer
+    // v is an integer-valued double (|v| < 2^100); returns v mod m in [-m/2, m/2).
+    // Each step r <- r - m rint(r/m) is exact (fma of an exactly representable
+    // integer result); rint(r/m) is off by at most |r| 2^-52 / m, so the
+    // magnitude drops to below 2^(e - 44) per step and two steps reach |r| < 2m.
+    double r = v;
+#pragma unroll
+    for (int it = 0; it < 3; ++it) {
+        const double q = rint(r * inv_m);
+        r = fma(-q, (double)m, r);
+    }
     int ri = (int)r;
     const int half = m >> 1;
     if (ri >= half + (m & 1)) ri -= m;
@@ -324,8 +331,9 @@ __global__ __launch_bounds__(256) void plane_kernel(PlaneArgs p) {
 // ---------------------------------------------------------------------------
 // batched INT8 GEMM over K-contiguous planes: D_z = sum_{(a,b) in pairs(z)} A_a B_b^T
 // ---------------------------------------------------------------------------
-template <int BM, int BN, int BK, int WM, int WN>
-__global__ __launch_bounds__(64 * WM * WN) void i8_gemm_kernel(I8Args p) {
+template <int BM, int BN, int BK, int WM, int WN, int OCC>
+__global__ __launch_bounds__(64 * WM * WN) __attribute__((amdgpu_waves_per_eu(OCC)))
+void i8_gemm_kernel(I8Args p) {
     constexpr int NT = 64 * WM * WN;
     constexpr int RS = BK + 16;      // padded LDS row (bytes): 16-byte reads of 16 rows hit distinct banks
     constexpr int CPR = BK / 16;     // 16-byte chunks per row
@@ -574,13 +582,13 @@ hipError_t planes(const double* x, int64_t ld, int rows, int k, int kp, bool kco
 }
 }  // namespace
 
-template <int BM, int BN, int BK, int WM, int WN>
+template <int BM, int BN, int BK, int WM, int WN, int OCC>
 static hipError_t i8_launch_cfg(I8Args p, int nout, hipStream_t s) {
     p.tiles_m = (p.m + BM - 1) / BM;
     p.tiles_n = (p.n + BN - 1) / BN;
     if (p.kp % BK != 0) return hipErrorInvalidValue;
     const size_t lds = 2 * (size_t)(BM + BN) * (BK + 16);
-    auto kern = i8_gemm_kernel<BM, BN, BK, WM, WN>;
+    auto kern = i8_gemm_kernel<BM, BN, BK, WM, WN, OCC>;
     hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
                                        (int)lds);
     if (e != hipSuccess) return e;
@@ -592,10 +600,12 @@ static hipError_t i8_launch_cfg(I8Args p, int nout, hipStream_t s) {
 // INT8 tile configurations (BM x BN x BK, wave grid); kp must be a multiple of BK.
 hipError_t i8_gemm_launch(const I8Args& args, int nout, hipStream_t s, int tile) {
     switch (tile) {
-        case 0: return i8_launch_cfg<256, 256, 128, 2, 2>(args, nout, s);
-        case 1: return i8_launch_cfg<256, 128, 64, 2, 2>(args, nout, s);
-        case 2: return i8_launch_cfg<128, 128, 128, 2, 2>(args, nout, s);
-        case 3: return i8_launch_cfg<256, 256, 64, 2, 2>(args, nout, s);
+        case 0: return i8_launch_cfg<256, 256, 128, 2, 2, 1>(args, nout, s);
+        case 1: return i8_launch_cfg<256, 128, 64, 2, 2, 2>(args, nout, s);
+        case 2: return i8_launch_cfg<128, 128, 128, 2, 2, 2>(args, nout, s);
+        case 3: return i8_launch_cfg<256, 256, 64, 2, 2, 1>(args, nout, s);
+        case 4: return i8_launch_cfg<256, 256, 128, 4, 2, 2>(args, nout, s);
+        case 5: return i8_launch_cfg<256, 256, 64, 4, 2, 2>(args, nout, s);
         default: return hipErrorInvalidValue;
     }
 }
