@@ -40,6 +40,8 @@ def parse_args():
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     p.add_argument("--no-split", action="store_true", help="skip the split-precision leg")
     p.add_argument("--no-variants", action="store_true", help="skip the transpose/beta/SGEMM variant table")
+    p.add_argument("--no-grouped", action="store_true", help="skip the grouped uneven-GEMM leg")
+    p.add_argument("--no-level3", action="store_true", help="skip the DTRSM/DSYRK leg")
     p.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU-baseline sample time")
     return p.parse_args()
 
@@ -187,6 +189,22 @@ def main() -> int:
             result["split_int8"] = {"error": str(exc)}
             log(f"split leg failed: {exc}")
 
+    # ---------------- grouped uneven GEMM -----------------
+    if not args.no_grouped:
+        try:
+            result["grouped"] = grouped_leg(torch, stream, achieved)
+        except Exception as exc:
+            result["grouped"] = {"error": str(exc)}
+            log(f"grouped leg failed: {exc}")
+
+    # ---------------- DTRSM / DSYRK -----------------
+    if not args.no_level3:
+        try:
+            result["level3"] = level3_leg(torch, lib, n, stream)
+        except Exception as exc:
+            result["level3"] = {"error": str(exc)}
+            log(f"level3 leg failed: {exc}")
+
     # ---------------- CPU baseline -----------------
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_leg(n, ld, args.cpu_seconds)
@@ -256,6 +274,124 @@ def split_leg(torch, lib, A, B, n, ld, stream, plain_tf):
         out["form"] = f"Ozaki scheme II, {best[0]} INT8 residue GEMMs, INT32 accumulation"
         out["speedup_vs_plain"] = round(best[1]["tflops_fp64_equiv"] / plain["tflops_fp64_equiv"], 3)
         out["error_le_plain"] = True
+    return out
+
+
+def grouped_leg(torch, stream, plain_tf):
+    """4096 problems, M, N, K log-uniform in [16, 512] (seed 0), one persistent
+    graph-replayed launch per batch vs a loop of single-GEMM launches."""
+    from zynum_amd import grouped
+
+    out = {"workload": "4096 independent GEMMs, M,N,K log-uniform integers in [16,512] (seed 0), NN, "
+                       "ld = rows, alpha=1, beta=0, entries uniform [-1,1)"}
+    shapes = grouped.draw_shapes(4096, 0, 16, 512)
+    sh = stream.cuda_stream
+    for name, dt in (("f64", torch.float64), ("f32", torch.float32)):
+        b = grouped.Batch(torch, shapes, dt, seed=0)
+        plan = b.plan()
+        info = plan.info()
+        plan.graph_run(sh)
+        torch.cuda.synchronize()
+
+        def timed(fn, reps):
+            s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            s.record(stream)
+            for _ in range(reps):
+                fn()
+            e.record(stream)
+            torch.cuda.synchronize()
+            return s.elapsed_time(e) / reps
+
+        g_ms = timed(lambda: plan.graph_run(sh), 10)
+        l_ms = timed(lambda: b.loop(sh), 2)
+        r = {"flop": b.flops, "tiles": info["tiles"], "workgroups": info["workgroups"],
+             "graph_ms": round(g_ms, 4), "tflops": round(b.flops / (g_ms * 1e-3) / 1e12, 2),
+             "loop_ms": round(l_ms, 4), "loop_tflops": round(b.flops / (l_ms * 1e-3) / 1e12, 2),
+             "speedup_vs_loop": round(l_ms / g_ms, 2),
+             "hbm_gbs_algorithmic": round(b.bytes / (g_ms * 1e-3) / 1e9, 1)}
+        if name == "f64":
+            r["frac_of_large_dgemm"] = round(r["tflops"] / plain_tf, 3)
+        out[name] = r
+        plan.close()
+        del b
+        log(f"grouped {name}: {r}")
+    return out
+
+
+def level3_leg(torch, lib, n, stream):
+    """DTRSM Left/Lower/NoTrans/NonUnit N=8192 with 8192 right-hand sides
+    (diagonal 8 + |u|) and DSYRK Lower NoTrans N=8192, K=4096, beta=0."""
+    from zynum_amd import _lib, synth
+
+    dev = torch.device("cuda:0")
+    sh = stream.cuda_stream
+    out = {}
+    # --- DTRSM ---
+    L = torch.from_numpy(synth.matrix(11, 0, n, n, n)).to(dev).view(n, n)  # row j = column j
+    L = torch.triu(L)  # column-major lower triangle == row-major upper
+    idx = torch.arange(n, device=dev)
+    L[idx, idx] = 8.0 + torch.from_numpy(synth.uniform(11, n * n, n)).to(dev).abs()
+    B = torch.from_numpy(synth.matrix(11, n * n + n, n, n, n)).to(dev)
+    flop = float(n) * n * n
+    res = {"workload": f"DTRSM L/L/N/N M=N={n}, ld={n}, alpha=1, recursive blocking with 64-row FP64 leaf solves"}
+    for mode, param, tag in ((_lib.MODE_PLAIN, 0, "plain_fp64"), (_lib.MODE_OZAKI_CRT, 14, "crt14_updates")):
+        X = B.clone()
+        st = lib.zynum_gpu_dtrsm(b"L", b"L", b"N", b"N", n, n, 1.0, L.data_ptr(), n, X.data_ptr(), n, sh, mode,
+                                 param, 0)
+        _lib.check(st, "dtrsm")
+        X = B.clone()
+        torch.cuda.synchronize()
+        s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        s.record(stream)
+        st = lib.zynum_gpu_dtrsm(b"L", b"L", b"N", b"N", n, n, 1.0, L.data_ptr(), n, X.data_ptr(), n, sh, mode,
+                                 param, 0)
+        e.record(stream)
+        torch.cuda.synchronize()
+        _lib.check(st, "dtrsm")
+        ms = s.elapsed_time(e)
+        # residual L X - B with the plain FP64 GEMM (column-major L is the transposed row-major tensor)
+        R = B.clone()
+        from zynum_amd import blas
+        st = blas.gemm_request(False, False, n, n, n, 1.0, L.data_ptr(), n, X.data_ptr(), n, -1.0, R.data_ptr(), n,
+                               _lib.F64, stream=sh)
+        _lib.check(st, "residual")
+        torch.cuda.synchronize()
+        nrm = float(L.abs().sum(dim=1).max()) * float(X.abs().max())
+        res[tag] = {"ms": round(ms, 3), "tflops": round(flop / (ms * 1e-3) / 1e12, 2),
+                    "residual_max": float(R.abs().max()), "residual_normwise": float(R.abs().max()) / nrm}
+        del X, R
+    out["dtrsm"] = res
+    del L, B
+    # --- DSYRK ---
+    k = n // 2
+    A = torch.from_numpy(synth.matrix(12, 0, n, k, n)).to(dev)
+    ref_hi = torch.zeros(n * n, dtype=torch.float64, device=dev)
+    ref_lo = torch.zeros(n * n, dtype=torch.float64, device=dev)
+    st = lib.zynum_gpu_dd_gemm(b"N", b"T", n, n, k, 1.0, A.data_ptr(), n, A.data_ptr(), n, 0.0, None, n,
+                               ref_hi.data_ptr(), ref_lo.data_ptr(), n, sh)
+    _lib.check(st, "dd syrk reference")
+    flop = float(n) * n * k
+    res = {"workload": f"DSYRK Lower NoTrans N={n} K={k}, beta=0, lower-triangle tiles of the GEMM core"}
+    low = torch.tril(torch.ones(n, n, dtype=torch.bool, device=dev)).T  # column-major lower in row-major view
+    for mode, param, tag in ((_lib.MODE_PLAIN, 0, "plain_fp64"), (_lib.MODE_OZAKI_CRT, 14, "crt14")):
+        C = torch.zeros(n * n, dtype=torch.float64, device=dev)
+        st = lib.zynum_gpu_dsyrk(b"L", b"N", n, k, 1.0, A.data_ptr(), n, 0.0, C.data_ptr(), n, sh, mode, param, 0)
+        _lib.check(st, "dsyrk")
+        torch.cuda.synchronize()
+        s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        s.record(stream)
+        for _ in range(3):
+            st = lib.zynum_gpu_dsyrk(b"L", b"N", n, k, 1.0, A.data_ptr(), n, 0.0, C.data_ptr(), n, sh, mode, param, 0)
+        e.record(stream)
+        torch.cuda.synchronize()
+        _lib.check(st, "dsyrk")
+        ms = s.elapsed_time(e) / 3
+        err = ((C - ref_hi) - ref_lo).view(n, n)[low].abs().max()
+        res[tag] = {"ms": round(ms, 3), "tflops": round(flop / (ms * 1e-3) / 1e12, 2),
+                    "max_abs_err_vs_dd": float(err)}
+        del C
+    out["dsyrk"] = res
+    log(f"level3: {out}")
     return out
 
 

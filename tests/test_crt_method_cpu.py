@@ -1,0 +1,87 @@
+"""The CRT reconstruction used by the Ozaki-II kernel, emulated on the CPU.
+
+The device kernel rebuilds C' from residues u_l as digits D_d = S_d - q P_d of
+40 bits (each an exact integer below 2^53), normalizes them to balanced digits
+with exact carries, and sums from the lowest digit.  This test emulates that
+arithmetic (exact integers where the kernel's steps are exact, IEEE doubles
+where it rounds) and checks the property the kernel relies on: the result is
+exactly C' whenever C' is a double, and within one rounding otherwise.  It also
+shows that summing the raw digit differences without carry normalization
+(a plausible mistake) loses exactness.
+"""
+import math
+import random
+
+MODULI = [256, 255, 253, 251, 247, 241, 239, 233, 229, 227, 223, 217, 211, 199, 197, 193, 191, 181, 179, 173]
+
+
+def setup(nmod):
+    P = 1
+    for m in MODULI[:nmod]:
+        P *= m
+    nd = (P.bit_length() + 8 + 39) // 40
+    e = [40 * (nd - 1 - d) for d in range(nd)]
+    dig = lambda x, lo: (x >> lo) & ((1 << 40) - 1)  # noqa: E731
+    h = [[dig(P // m, ed) for ed in e] for m in MODULI[:nmod]]
+    y = [pow((P // m) % m, -1, m) for m in MODULI[:nmod]]
+    p = [dig(P, ed) for ed in e]
+    papprox = sum(math.ldexp(float(p[d]), e[d]) for d in range(nd))
+    inv = [math.ldexp(1.0, e[d]) / papprox for d in range(nd)]
+    return P, nd, e, h, y, p, inv
+
+
+def reconstruct(C, nmod, cfg, normalize=True):
+    P, nd, e, h, y, p, inv = cfg
+    u = [(y[l] * (C % MODULI[l])) % MODULI[l] for l in range(nmod)]
+    S = [sum(h[l][d] * u[l] for l in range(nmod)) for d in range(nd)]
+    assert all(s < 2**53 for s in S)
+    xq = 0.0
+    for d in range(nd):
+        xq = float(S[d]) * inv[d] + xq
+    q = round(xq)
+    D = [S[d] - q * p[d] for d in range(nd)]
+    assert all(abs(x) < 2**53 for x in D)
+    if normalize:
+        for d in range(nd - 1, 0, -1):
+            c = round(D[d] * 2.0**-40)
+            D[d] -= c * 2**40
+            D[d - 1] += c
+    cp = 0.0
+    for d in range(nd - 1, -1, -1):
+        cp = math.ldexp(float(D[d]), e[d]) + cp
+    return cp
+
+
+def representable(rng, P):
+    sh = rng.randint(0, max(0, P.bit_length() - 60))
+    C = rng.randint(-2**52, 2**52) << sh
+    while abs(C) >= P // 2:
+        C //= 4
+    return C
+
+
+def test_reconstruction_exact_when_representable():
+    rng = random.Random(1)
+    for nmod in (8, 14, 16, 20):
+        cfg = setup(nmod)
+        for _ in range(3000):
+            C = representable(rng, cfg[0])
+            assert reconstruct(C, nmod, cfg) == float(C)
+
+
+def test_reconstruction_one_rounding_otherwise():
+    rng = random.Random(2)
+    for nmod in (10, 14, 18):
+        cfg = setup(nmod)
+        P = cfg[0]
+        for _ in range(3000):
+            C = rng.randint(-(P // 2) + 1, P // 2 - 1)
+            assert abs(reconstruct(C, nmod, cfg) - C) <= abs(C) * 2.0**-52
+
+
+def test_unnormalized_digits_lose_exactness():
+    rng = random.Random(3)
+    cfg = setup(14)
+    bad = sum(reconstruct(C, 14, cfg, normalize=False) != float(C)
+              for C in (representable(rng, cfg[0]) for _ in range(2000)))
+    assert bad > 0

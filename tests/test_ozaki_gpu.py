@@ -149,3 +149,33 @@ def test_slices_error_decreases_with_slices(torch_cuda):
         errs.append(np.max(np.abs((got - hi) - lo)))
     for e0, e1 in zip(errs, errs[1:]):
         assert e1 < e0 / 16
+
+
+def mismatch_report(got, want):
+    bad = np.argwhere(got != want)
+    if bad.size == 0:
+        return ""
+    idx = [tuple(x) for x in bad[:8]]
+    vals = [(got[i], want[i]) for i in idx]
+    return f"{len(bad)} mismatches of {got.size}; rows {sorted(set(int(x[0]) for x in bad))[:12]} " \
+           f"cols {sorted(set(int(x[1]) for x in bad))[:12]}; first {idx} {vals}"
+
+
+@pytest.mark.parametrize("shape", [(301, 301, 77), (108, 64, 192), (64, 64, 128), (300, 64, 64), (257, 129, 130),
+                                   (512, 384, 77)])
+@pytest.mark.parametrize("ta,tb", [("N", "T"), ("N", "N"), ("T", "N")])
+def test_crt_integer_exact_shapes(torch_cuda, shape, ta, tb):
+    """CRT exactness across tile counts, K below/above one 128-deep slice and
+    ragged edges (diagnostic message lists the mismatching rows/columns)."""
+    m, n, k = shape
+    rng = np.random.default_rng(m * 7 + n * 3 + k)
+    A = rng.integers(-9, 10, (m, k)).astype(np.float64)
+    B = rng.integers(-9, 10, (k, n)).astype(np.float64)
+    a = (A if ta == "N" else A.T).T.ravel()  # column-major storage of the stored matrix
+    b = (B if tb == "N" else B.T).T.ravel()
+    lda = m if ta == "N" else k
+    ldb = k if tb == "N" else n
+    c = np.zeros(m * n)
+    got = run(torch_cuda, ta, tb, m, n, k, 1.0, a, lda, b, ldb, 0.0, c, m, 3, 14).reshape(n, m).T
+    want = (A.astype(np.int64) @ B.astype(np.int64)).astype(np.float64)
+    assert np.array_equal(got, want), mismatch_report(got, want)

@@ -514,10 +514,25 @@ __global__ __launch_bounds__(256) void crt_reconstruct_kernel(CrtArgs p) {
         for (int d = 0; d < kMaxDigits; ++d)
             if (d < nd) xq = fma(S[r][d], p.cc.inv_p_dig[d], xq);
         const double q = rint(xq);
+        // C' = sum_d D_d 2^(40 (nd-1-d)) with D_d = S_d - q P_d (exact, |D_d| < 2^53).
+        // Normalize to balanced 40-bit digits with exact carries, so that every
+        // lower part is below half an ulp of what is added on top of it; the sum
+        // is then within one rounding of C' and exact when C' is representable.
+        double D[kMaxDigits];
+#pragma unroll
+        for (int d = 0; d < kMaxDigits; ++d) D[d] = (d < nd) ? fma(-q, p.cc.p[d], S[r][d]) : 0.0;
+#pragma unroll
+        for (int d = kMaxDigits - 1; d >= 1; --d) {
+            if (d < nd) {
+                const double carry = rint(D[d] * 0x1p-40);
+                D[d] = fma(-carry, 0x1p40, D[d]);  // exact, |D_d| <= 2^39
+                D[d - 1] += carry;                 // exact, integers well below 2^53
+            }
+        }
         double cp = 0.0;
 #pragma unroll
-        for (int d = 0; d < kMaxDigits; ++d)
-            if (d < nd) cp += ldexp(fma(-q, p.cc.p[d], S[r][d]), p.cc.dig_exp[d]);  // exact digit differences
+        for (int d = kMaxDigits - 1; d >= 0; --d)
+            if (d < nd) cp = ldexp(D[d], p.cc.dig_exp[d]) + cp;
         const double prod = ldexp(cp, -(p.sig_a[i] + sb));
         double* cptr = p.c + (int64_t)j * p.ldc + i;
         *cptr = epilogue<double>(p.alpha, p.beta, prod, cptr);

@@ -40,10 +40,10 @@ class Batch:
         for (m, nn, k) in self.shapes:
             ra, ca = (k, m) if transa else (m, k)
             rb, cb = (nn, k) if transb else (k, nn)
-            lda, ldb, ldc = ra + pad, rb + pad, m + pad
+            lda, ldb, ldc = max(1, ra) + pad, max(1, rb) + pad, max(1, m) + pad
             offs_a.append(ta); offs_b.append(tb); offs_c.append(tc)
-            ta += lda * ca; tb += ldb * cb; tc += ldc * nn
-            lds.append((lda, ldb, ldc))
+            ta += int(lda * ca); tb += int(ldb * cb); tc += int(ldc * nn)
+            lds.append((int(lda), int(ldb), int(ldc)))
         g = torch.Generator(device=device).manual_seed(seed)
         self.A = torch.rand(max(1, ta), generator=g, device=device, dtype=dtype) * 2 - 1
         self.B = torch.rand(max(1, tb), generator=g, device=device, dtype=dtype) * 2 - 1
@@ -56,11 +56,11 @@ class Batch:
             p.transa = _lib.TRANS if transa else _lib.NO_TRANS
             p.transb = _lib.TRANS if transb else _lib.NO_TRANS
             p.m, p.n, p.k = int(m), int(nn), int(k)
-            p.lda, p.ldb, p.ldc = lds[i]
+            p.lda, p.ldb, p.ldc = (int(x) for x in lds[i])
             p.alpha, p.beta = alpha, beta
-            p.a = self.A.data_ptr() + offs_a[i] * self.es
-            p.b = self.B.data_ptr() + offs_b[i] * self.es
-            p.c = self.C.data_ptr() + offs_c[i] * self.es
+            p.a = int(self.A.data_ptr() + int(offs_a[i]) * self.es)
+            p.b = int(self.B.data_ptr() + int(offs_b[i]) * self.es)
+            p.c = int(self.C.data_ptr() + int(offs_c[i]) * self.es)
         self.flops = float(np.sum(2.0 * self.shapes[:, 0] * self.shapes[:, 1] * self.shapes[:, 2]))
         self.bytes = float(np.sum(self.es * (self.shapes[:, 0] * self.shapes[:, 2] + self.shapes[:, 2] * self.shapes[:, 1]
                                              + self.shapes[:, 0] * self.shapes[:, 1] * (2 if beta != 0 else 1))))
