@@ -43,6 +43,10 @@ def parse_args():
     p.add_argument("--no-grouped", action="store_true", help="skip the grouped uneven-GEMM leg")
     p.add_argument("--no-level3", action="store_true", help="skip the DTRSM/DSYRK leg")
     p.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU-baseline sample time")
+    p.add_argument("--only-headline", action="store_true", help="plain FP64 leg only (for profiler runs)")
+    p.add_argument("--calibrate-copy", action="store_true",
+                   help="before the workload, read 4 GiB once and copy 1 GiB once (known HBM bytes for "
+                        "scaling the FETCH_SIZE / WRITE_SIZE counters)")
     return p.parse_args()
 
 
@@ -58,6 +62,8 @@ def read_traffic(name: str):
 
 def main() -> int:
     args = parse_args()
+    if args.only_headline:
+        args.no_cpu = args.no_split = args.no_variants = args.no_grouped = args.no_level3 = True
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print(json.dumps({"error": "this build runs on one GPU only; multi-GPU sharding is a design note, "
@@ -75,6 +81,17 @@ def main() -> int:
     n = args.size
     ld = n + 8
     flop = 2.0 * n * n * n
+    if args.calibrate_copy:
+        big = torch.ones(1 << 29, dtype=torch.float64, device=dev)  # 4 GiB, larger than the 256 MB MALL
+        flush = torch.ones(1 << 27, dtype=torch.float64, device=dev)
+        flush.mul_(2.0)  # evict big from the caches
+        log(f"calibration read sum {float(big.sum()):.0f} (4 GiB read)")
+        x = torch.ones(1 << 27, dtype=torch.float64, device=dev)
+        y = torch.empty_like(x)
+        flush.mul_(2.0)
+        y.copy_(x)  # 1 GiB read + 1 GiB write
+        torch.cuda.synchronize()
+        del big, flush, x, y
 
     log(f"inputs: DGEMM {n}^3 NN ld={ld}, splitmix64 seed 0")
     t0 = time.time()
@@ -259,7 +276,7 @@ def split_leg(torch, lib, A, B, n, ld, stream, plain_tf):
            "plain_fp64": plain}
     best = None
     for moduli in (14, 15, 16):
-        r = run(_lib.MODE_OZAKI_CRT, moduli, 3)
+        r = run(_lib.MODE_OZAKI_CRT, moduli, 5)
         r["int8_gemms"] = moduli
         r["int8_tops_executed"] = round(moduli * flop / (r["ms"] * 1e-3) / 1e12, 1)
         out[f"crt_{moduli}"] = r
@@ -267,7 +284,16 @@ def split_leg(torch, lib, A, B, n, ld, stream, plain_tf):
             best = (moduli, r)
     for slices in (4, 5, 6, 7, 8):
         r = run(_lib.MODE_OZAKI_INT8, slices, 1)
+        pairs = slices * (slices + 1) // 2  # slice products with i + j <= s + 1
+        r["int8_slice_products"] = pairs
+        r["int8_tops_executed"] = round(pairs * flop / (r["ms"] * 1e-3) / 1e12, 1)
         out[f"slices_int8_{slices}"] = r
+    for slices in (6, 7, 8):
+        r = run(_lib.MODE_OZAKI_BF16, slices, 1)
+        pairs = slices * (slices + 1) // 2
+        r["bf16_slice_products"] = pairs
+        r["bf16_tflops_executed"] = round(pairs * flop / (r["ms"] * 1e-3) / 1e12, 1)
+        out[f"slices_bf16_{slices}"] = r
     if best is not None:
         out["value"] = best[1]["tflops_fp64_equiv"]
         out["unit"] = "TFLOP/s (FP64-equivalent 2MNK)"

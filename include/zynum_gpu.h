@@ -139,8 +139,8 @@ typedef struct zynum_gpu_gemm_request {
     uint8_t mode;        /* ZYNUM_GPU_MODE_* */
     uint8_t mode_param;  /* slices or moduli count; 0 picks the default */
     uint8_t flags;       /* ZYNUM_GPU_FLAG_* */
-    uint8_t tile;        /* tile configuration of the GEMM core (plain: FP64/FP32 MFMA tiles, split modes: INT8
-                            tiles): 0 = tuned choice, i + 1 = configuration i */
+    uint8_t tile;        /* tile configuration of the GEMM core: 0 = tuned choice, i + 1 = configuration i
+                            (plain: FP64/FP32 MFMA tiles 0..7; split modes: INT8 tiles 0..9) */
     uint8_t reserved1[4];
 } zynum_gpu_gemm_request; /* 104 bytes */
 

@@ -19,7 +19,8 @@ pytestmark = pytest.mark.gpu
 GOLDEN = Path(__file__).resolve().parent / "golden"
 
 
-def gpu_gemm(torch, ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, mode=0, mode_param=0, via="request"):
+def gpu_gemm(torch, ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, mode=0, mode_param=0, via="request",
+             tile=0):
     """Run one GEMM on device copies of the numpy operands; returns C (numpy)."""
     from zynum_amd import _lib, blas
 
@@ -28,7 +29,7 @@ def gpu_gemm(torch, ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, mode=0
     scalar = _lib.F64 if dt == np.float64 else _lib.F32
     if via == "request":
         st = blas.gemm_request(ta != "N", tb != "N", m, n, k, alpha, da.data_ptr(), lda, db.data_ptr(), ldb, beta,
-                               dc.data_ptr(), ldc, scalar, mode=mode, mode_param=mode_param)
+                               dc.data_ptr(), ldc, scalar, mode=mode, mode_param=mode_param, tile=tile)
         _lib.check(st, "gemm")
     else:
         lib = _lib.load()
@@ -241,3 +242,32 @@ def test_8192_padded_integer_sampled(torch_cuda, dtype, ta, tb, beta):
     got = Cm[cols, rows].to(torch.float64)
     assert torch.equal(got, exact)
     assert torch.all(Cm[:, n:] == -999.0)
+
+
+PLAIN_TILES = 8
+
+
+@pytest.mark.parametrize("tile", range(1, PLAIN_TILES + 1))
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("ta,tb", [("N", "N"), ("T", "T")])
+def test_every_tile_config_bitwise(torch_cuda, tile, dtype, ta, tb):
+    """Every FP64/FP32 tile configuration gives the exact product on integer
+    inputs over a ragged shape spanning several tiles (padded ld, alpha/beta),
+    so the tuned tile choice cannot change results."""
+    m, n, k = 301, 263, 150
+    lda = (m if ta == "N" else k) + 3
+    ldb = (k if tb == "N" else n) + 1
+    ldc = m + 2
+    a, b, c = operands(ta, tb, m, n, k, lda, ldb, ldc, dtype, True, salts=(5, 9, 2))
+    want = int_gemm(ta, tb, m, n, k, -0.5, a, lda, b, ldb, 2.0, c, ldc)
+    got = gpu_gemm(torch_cuda, ta, tb, m, n, k, -0.5, a, lda, b, ldb, 2.0, c, ldc, tile=tile)
+    assert np.array_equal(got, want.astype(dtype))
+
+
+def test_plain_tile_past_last_declined(torch_cuda):
+    from zynum_amd import _lib, blas
+
+    a = torch_cuda.ones(64 * 64, dtype=torch_cuda.float64, device="cuda")
+    st = blas.gemm_request(False, False, 64, 64, 64, 1.0, a.data_ptr(), 64, a.data_ptr(), 64, 0.0, a.data_ptr(), 64,
+                           _lib.F64, tile=PLAIN_TILES + 1)
+    assert st == _lib.DECLINE_UNSUPPORTED

@@ -19,12 +19,12 @@ pytestmark = pytest.mark.gpu
 MODULI = [256, 255, 253, 251, 247, 241, 239, 233, 229, 227, 223, 217, 211, 199, 197, 193, 191, 181, 179, 173]
 
 
-def run(torch, ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, mode, param):
+def run(torch, ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, mode, param, tile=0):
     from zynum_amd import _lib, blas
 
     da, db, dc = (torch.from_numpy(x.copy()).cuda() for x in (a, b, c))
     st = blas.gemm_request(ta != "N", tb != "N", m, n, k, alpha, da.data_ptr(), lda, db.data_ptr(), ldb, beta,
-                           dc.data_ptr(), ldc, _lib.F64, mode=mode, mode_param=param)
+                           dc.data_ptr(), ldc, _lib.F64, mode=mode, mode_param=param, tile=tile)
     _lib.check(st, f"mode {mode}/{param}")
     torch.cuda.synchronize()
     return dc.cpu().numpy()
@@ -65,7 +65,7 @@ def test_dd_gemm_matches_cpu_double_double(torch_cuda, oracle):
     assert np.all(diff <= 2.0**-98 * (1.5 * mag[rows, cols] + 1))
 
 
-@pytest.mark.parametrize("mode,param", [(3, 14), (3, 8), (3, 20), (1, 2), (1, 4), (1, 7)])
+@pytest.mark.parametrize("mode,param", [(3, 14), (3, 8), (3, 20), (1, 2), (1, 4), (1, 7), (2, 3), (2, 7)])
 @pytest.mark.parametrize("ta,tb", [("N", "N"), ("N", "T"), ("T", "N"), ("T", "T")])
 def test_integer_inputs_exact(torch_cuda, mode, param, ta, tb):
     """Integer inputs in [-14, 14]: power-of-two scaling, slicing and residues
@@ -179,3 +179,81 @@ def test_crt_integer_exact_shapes(torch_cuda, shape, ta, tb):
     got = run(torch_cuda, ta, tb, m, n, k, 1.0, a, lda, b, ldb, 0.0, c, m, 3, 14).reshape(n, m).T
     want = (A.astype(np.int64) @ B.astype(np.int64)).astype(np.float64)
     assert np.array_equal(got, want), mismatch_report(got, want)
+
+
+I8_TILES = 10
+BF16_TILES = 3
+
+
+@pytest.mark.parametrize("tile", range(1, I8_TILES + 1))
+@pytest.mark.parametrize("mode,param", [(3, 14), (1, 4)])
+def test_every_int8_tile_exact(torch_cuda, tile, mode, param):
+    """Every INT8 tile configuration (16x16x64 and 32x32x32 MFMA forms, BK 64
+    and 128) gives the exact integer product on ragged shapes spanning several
+    tiles in both directions; a configuration past the last one is declined."""
+    m, n, k = 523, 389, 300
+    rng = np.random.default_rng(tile * 10 + mode)
+    A = rng.integers(-9, 10, (m, k)).astype(np.float64)
+    B = rng.integers(-9, 10, (k, n)).astype(np.float64)
+    c = rng.integers(-9, 10, m * n).astype(np.float64)
+    got = run(torch_cuda, "N", "N", m, n, k, 1.0, A.T.ravel(), m, B.T.ravel(), k, 2.0, c, m, mode, param,
+              tile).reshape(n, m).T
+    want = (A.astype(np.int64) @ B.astype(np.int64)).astype(np.float64) + 2.0 * c.reshape(n, m).T
+    assert np.array_equal(got, want), mismatch_report(got, want)
+
+
+def test_int8_tile_past_last_declined(torch_cuda):
+    from zynum_amd import _lib, blas
+
+    a = torch_cuda.ones(64 * 64, dtype=torch_cuda.float64, device="cuda")
+    st = blas.gemm_request(False, False, 64, 64, 64, 1.0, a.data_ptr(), 64, a.data_ptr(), 64, 0.0, a.data_ptr(), 64,
+                           _lib.F64, mode=3, mode_param=14, tile=I8_TILES + 1)
+    assert st == _lib.DECLINE_UNSUPPORTED
+
+
+@pytest.mark.parametrize("tile", range(0, BF16_TILES + 1))
+def test_bf16_slices_bitwise_equal_int8_slices(torch_cuda, tile):
+    """BF16 slices hold the same integers as INT8 slices, FP32 sums of up to
+    4096 slice products are exact and are flushed into FP64, so the BF16 form
+    equals the INT8 form bitwise on any input (K = 4500 crosses a flush)."""
+    m, n, k = 300, 260, 4500
+    rng = np.random.default_rng(40 + tile)
+    a = rng.uniform(-1, 1, m * k) * np.exp2(rng.integers(-16, 17, m * k))
+    b = rng.uniform(-1, 1, k * n) * np.exp2(rng.integers(-16, 17, k * n))
+    c = rng.uniform(-1, 1, m * n)
+    want = run(torch_cuda, "N", "T", m, n, k, 1.25, a, m, b, n, 0.5, c, m, 1, 6)
+    got = run(torch_cuda, "N", "T", m, n, k, 1.25, a, m, b, n, 0.5, c, m, 2, 6, tile)
+    assert np.array_equal(got, want), mismatch_report(got.reshape(n, m), want.reshape(n, m))
+
+
+def test_bf16_tile_past_last_declined(torch_cuda):
+    from zynum_amd import _lib, blas
+
+    a = torch_cuda.ones(64 * 64, dtype=torch_cuda.float64, device="cuda")
+    st = blas.gemm_request(False, False, 64, 64, 64, 1.0, a.data_ptr(), 64, a.data_ptr(), 64, 0.0, a.data_ptr(), 64,
+                           _lib.F64, mode=2, mode_param=4, tile=BF16_TILES + 1)
+    assert st == _lib.DECLINE_UNSUPPORTED
+
+
+def test_k_limits_of_integer_accumulation(torch_cuda):
+    """INT32 headroom: CRT declines K >= 2^17 and s-slice INT8 declines
+    s K >= 2^19 (before writing C); BF16 slices flush FP32 into FP64 and stay
+    exact at K = 2^16 + 3 with every slice product at its maximum."""
+    from zynum_amd import _lib
+
+    m, n = 4, 3
+    for mode, param, k in ((3, 14, 1 << 17), (1, 8, 1 << 16)):
+        a = np.ones(m * k)
+        b = np.ones(k * n)
+        c = np.full(m * n, 7.0)
+        with pytest.raises(_lib.ZynumGpuError) as ei:
+            run(torch_cuda, "N", "N", m, n, k, 1.0, a, m, b, k, 0.0, c, m, mode, param)
+        assert ei.value.code == _lib.DECLINE_UNSUPPORTED
+    k = (1 << 16) + 3
+    x = 127.0 / 128.0  # slices 64, -64: every slice product at the 2^12 maximum magnitude
+    a = np.full(m * k, x)
+    b = np.full(k * n, x)
+    want = float(k) * x * x  # exact: k * 16129 / 2^14
+    for mode, param in ((2, 8), (1, 7)):
+        got = run(torch_cuda, "N", "N", m, n, k, 1.0, a, m, b, k, 0.0, np.zeros(m * n), m, mode, param)
+        assert np.all(got == want), (mode, got[:3], want)

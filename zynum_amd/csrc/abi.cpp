@@ -13,6 +13,7 @@
 
 #include "gemm.hpp"
 #include "ops.hpp"
+#include "ozaki.hpp"
 #include "runtime.hpp"
 
 namespace {
@@ -110,7 +111,7 @@ int gemm_op(bool ta, bool tb, int m, int n, int k, T alpha, const T* a, int64_t 
     if (take_forced_decline()) return ZYNUM_GPU_DECLINE_FORCED;
     if (!device_ready()) return ZYNUM_GPU_DECLINE_NO_DEVICE;
     if (mode != ZYNUM_GPU_MODE_PLAIN && sizeof(T) != 8) return ZYNUM_GPU_DECLINE_UNSUPPORTED;
-    if (mode > ZYNUM_GPU_MODE_DD) return ZYNUM_GPU_DECLINE_UNSUPPORTED;
+    if (mode < ZYNUM_GPU_MODE_PLAIN || mode > ZYNUM_GPU_MODE_DD) return ZYNUM_GPU_DECLINE_UNSUPPORTED;
 
     const bool reads_ab = (k > 0) && (alpha != T(0));
     Operand oa, ob, oc;
@@ -340,7 +341,10 @@ int zynum_gpu_gemm(const zynum_gpu_gemm_request* r) {
     const char tb = r->transb == ZYNUM_GPU_NO_TRANS ? 'N' : r->transb <= ZYNUM_GPU_CONJ_TRANS ? 'T' : 'X';
     if (gemm_error(ta, tb, r->m, r->n, r->k, r->lda, r->ldb, r->ldc) != 0) return ZYNUM_GPU_DECLINE_ARGUMENT;
     if (r->alpha_im != 0 || r->beta_im != 0) return ZYNUM_GPU_DECLINE_UNSUPPORTED;
-    if (r->tile > (r->mode == ZYNUM_GPU_MODE_PLAIN ? zg::kTileConfigs : 6)) return ZYNUM_GPU_DECLINE_UNSUPPORTED;
+    const int tiles = r->mode == ZYNUM_GPU_MODE_PLAIN       ? zg::kTileConfigs
+                      : r->mode == ZYNUM_GPU_MODE_OZAKI_BF16 ? zg::kBf16Tiles
+                                                             : zg::kI8Tiles;
+    if (r->tile > tiles) return ZYNUM_GPU_DECLINE_UNSUPPORTED;
     hipStream_t s = r->stream ? reinterpret_cast<hipStream_t>(r->stream) : zg::blas_stream();
     if (r->scalar == ZYNUM_GPU_F64) {
         return zg::gemm_op<double>(ta == 'T', tb == 'T', r->m, r->n, r->k, scalar_from_bits<double>(r->alpha_re),

@@ -241,6 +241,7 @@ struct PlaneArgs {
     int64_t plane_stride;  // bytes between planes
     int nplanes;
     int crt;               // 1: residues modulo cc.m[l]; 0: 7-bit slices
+    int bf16;              // slices stored as bf16 (2 bytes, exact: |s| <= 64) instead of int8
     int m[kMaxModuli];
     double inv_m[kMaxModuli];
 };
@@ -285,7 +286,8 @@ __global__ __launch_bounds__(256) void plane_kernel(PlaneArgs p) {
     const int gr = r0 + r;
     if (gr >= p.rows || k0 + kq >= p.kp) return;
     const int sg = p.sigma[gr];
-    int8_t* dst = p.out + (int64_t)gr * p.kp + k0 + kq;
+    const int eb = p.bf16 ? 2 : 1;
+    int8_t* dst = p.out + ((int64_t)gr * p.kp + k0 + kq) * eb;
     if (p.crt) {
         for (int l = 0; l < p.nplanes; ++l) {
             int4 packed;
@@ -310,20 +312,34 @@ __global__ __launch_bounds__(256) void plane_kernel(PlaneArgs p) {
         for (int b = 0; b < 16; ++b) rem[b] = ldexp(tile[r][kq + b], sg);  // |rem| < 1
         for (int l = 0; l < p.nplanes; ++l) {
             const int shift = (l == 0) ? 6 : 6 + 7 * l;
-            int4 packed;
-            int* pw = reinterpret_cast<int*>(&packed);
+            double v[16];
 #pragma unroll
-            for (int w = 0; w < 4; ++w) {
-                uint32_t word = 0;
-#pragma unroll
-                for (int b = 0; b < 4; ++b) {
-                    const double v = rint(ldexp(rem[4 * w + b], shift));
-                    rem[4 * w + b] = fma(-v, ldexp(1.0, -shift), rem[4 * w + b]);  // exact
-                    word |= ((uint32_t)(uint8_t)(int8_t)(int)v) << (8 * b);
-                }
-                pw[w] = (int)word;
+            for (int b = 0; b < 16; ++b) {
+                v[b] = rint(ldexp(rem[b], shift));
+                rem[b] = fma(-v[b], ldexp(1.0, -shift), rem[b]);  // exact
             }
-            *reinterpret_cast<int4*>(dst + (int64_t)l * p.plane_stride) = packed;
+            int8_t* d = dst + (int64_t)l * p.plane_stride;
+            if (p.bf16) {
+                // |v| <= 64: the float has at most 7 significant bits, so its top 16 bits are the bf16
+                int4 packed[2];
+                uint32_t* pw = reinterpret_cast<uint32_t*>(packed);
+#pragma unroll
+                for (int w = 0; w < 8; ++w)
+                    pw[w] = (__float_as_uint((float)v[2 * w]) >> 16) | (__float_as_uint((float)v[2 * w + 1]) & 0xffff0000u);
+                reinterpret_cast<int4*>(d)[0] = packed[0];
+                reinterpret_cast<int4*>(d)[1] = packed[1];
+            } else {
+                int4 packed;
+                uint32_t* pw = reinterpret_cast<uint32_t*>(&packed);
+#pragma unroll
+                for (int w = 0; w < 4; ++w) {
+                    uint32_t word = 0;
+#pragma unroll
+                    for (int b = 0; b < 4; ++b) word |= ((uint32_t)(uint8_t)(int8_t)(int)v[4 * w + b]) << (8 * b);
+                    pw[w] = word;
+                }
+                *reinterpret_cast<int4*>(d) = packed;
+            }
         }
     }
 }
@@ -331,18 +347,101 @@ __global__ __launch_bounds__(256) void plane_kernel(PlaneArgs p) {
 // ---------------------------------------------------------------------------
 // batched INT8 GEMM over K-contiguous planes: D_z = sum_{(a,b) in pairs(z)} A_a B_b^T
 // ---------------------------------------------------------------------------
-template <int BM, int BN, int BK, int WM, int WN, int OCC>
+// MFMA shapes: MS = 16 -> v_mfma_i32_16x16x64_i8 (4 results per lane: rows
+// 4*(lane/16)+r of column lane%16); MS = 32 -> v_mfma_i32_32x32x32_i8 (16
+// results per lane: rows 8*(r/4)+4*(lane/32)+r%4 of column lane%32).  Each
+// lane feeds 16 consecutive k of one row of A and of B; the k order inside
+// the instruction is the same for both operands, so the sum is the plain dot
+// product whatever it is.
+template <int MS>
+struct I8Mfma;
+template <>
+struct I8Mfma<16> {
+    static constexpr int KD = 64, NR = 4, MSZ = 16;
+    static constexpr bool kFloat = false;
+    typedef v4i acc_t;
+    static __device__ __forceinline__ acc_t zero() { return acc_t{0, 0, 0, 0}; }
+    static __device__ __forceinline__ acc_t mma(v4i a, v4i b, acc_t c) {
+        return __builtin_amdgcn_mfma_i32_16x16x64_i8(a, b, c, 0, 0, 0);
+    }
+    // row offset of result group g (4 consecutive rows) for this lane
+    static __device__ __forceinline__ int group_row(int lane, int) { return (lane >> 4) * 4; }
+};
+template <>
+struct I8Mfma<32> {
+    static constexpr int KD = 32, NR = 16, MSZ = 32;
+    static constexpr bool kFloat = false;
+    typedef int acc_t __attribute__((ext_vector_type(16)));
+    static __device__ __forceinline__ acc_t zero() {
+        acc_t z;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) z[r] = 0;
+        return z;
+    }
+    static __device__ __forceinline__ acc_t mma(v4i a, v4i b, acc_t c) {
+        return __builtin_amdgcn_mfma_i32_32x32x32_i8(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int group_row(int lane, int g) { return 8 * g + 4 * (lane >> 5); }
+};
+
+// BF16 forms over the same byte layout: 16 bytes per lane are 8 bf16 values,
+// so v_mfma_f32_16x16x32_bf16 reads what v_mfma_i32_16x16x64_i8 reads (64
+// bytes of k per row) and v_mfma_f32_32x32x16_bf16 what the 32x32x32 INT8
+// form reads; the result layouts are the same as the INT8 forms'.
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+template <int MS>
+struct Bf16Mfma;
+template <>
+struct Bf16Mfma<16> {
+    static constexpr int KD = 64, NR = 4, MSZ = 16;
+    static constexpr bool kFloat = true;
+    typedef float acc_t __attribute__((ext_vector_type(4)));
+    static __device__ __forceinline__ acc_t zero() { return acc_t{0.f, 0.f, 0.f, 0.f}; }
+    static __device__ __forceinline__ acc_t mma(v4i a, v4i b, acc_t c) {
+        return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(v8bf, a), __builtin_bit_cast(v8bf, b), c,
+                                                       0, 0, 0);
+    }
+    static __device__ __forceinline__ int group_row(int lane, int) { return (lane >> 4) * 4; }
+};
+template <>
+struct Bf16Mfma<32> {
+    static constexpr int KD = 32, NR = 16, MSZ = 32;
+    static constexpr bool kFloat = true;
+    typedef float acc_t __attribute__((ext_vector_type(16)));
+    static __device__ __forceinline__ acc_t zero() {
+        acc_t z;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) z[r] = 0.f;
+        return z;
+    }
+    static __device__ __forceinline__ acc_t mma(v4i a, v4i b, acc_t c) {
+        return __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(v8bf, a), __builtin_bit_cast(v8bf, b), c,
+                                                       0, 0, 0);
+    }
+    static __device__ __forceinline__ int group_row(int lane, int g) { return 8 * g + 4 * (lane >> 5); }
+};
+
+// BK is in bytes (p.kp is the plane row length in bytes).  For the BF16 forms
+// every product of two slices is an integer of at most 2^12 in magnitude, so
+// FP32 sums of up to 4096 of them are exact; the FP32 accumulators are added
+// into FP64 ones every kFlush steps (BK/2 products per step) and cleared.
+template <int BM, int BN, int BK, int WM, int WN, int OCC, class MF>
 __global__ __launch_bounds__(64 * WM * WN) __attribute__((amdgpu_waves_per_eu(OCC)))
 void i8_gemm_kernel(I8Args p) {
+    constexpr int MS = MF::MSZ;
+    constexpr bool FL = MF::kFloat;
+    constexpr int kFlush = 8192 / BK;
+    typedef typename MF::acc_t acc_t;
     constexpr int NT = 64 * WM * WN;
-    constexpr int RS = BK + 16;      // padded LDS row (bytes): 16-byte reads of 16 rows hit distinct banks
+    constexpr int RS = BK + 16;      // padded LDS row (bytes): 16-byte reads of consecutive rows hit distinct banks
     constexpr int CPR = BK / 16;     // 16-byte chunks per row
     constexpr int WTM = BM / WM, WTN = BN / WN;
-    constexpr int FM = WTM / 16, FN = WTN / 16;
+    constexpr int FM = WTM / MS, FN = WTN / MS;
     constexpr int LA = BM * CPR / NT;  // 16-byte loads per thread (A)
     constexpr int LB = BN * CPR / NT;
+    constexpr int NG = MF::NR / 4;     // groups of 4 consecutive result rows per lane
     static_assert(LA >= 1 && LB >= 1 && BM * CPR % NT == 0 && BN * CPR % NT == 0, "tile/workgroup mismatch");
-    static_assert(BK % 64 == 0, "BK is a multiple of the MFMA depth");
+    static_assert(BK % 64 == 0 && WTM % MS == 0 && WTN % MS == 0, "tile is a multiple of the MFMA shape");
 
     extern __shared__ __attribute__((aligned(16))) int8_t lds[];
     // stage buf: A rows at lds + buf*(BM+BN)*RS, B rows right after them
@@ -357,11 +456,32 @@ void i8_gemm_kernel(I8Args p) {
     const int m0 = tm * BM, n0 = tn * BN;
     const I8Out& out = p.outs[blockIdx.z];
 
-    v4i acc[FM][FN];
+    acc_t acc[FM][FN];
 #pragma unroll
     for (int i = 0; i < FM; ++i)
 #pragma unroll
-        for (int j = 0; j < FN; ++j) acc[i][j] = v4i{0, 0, 0, 0};
+        for (int j = 0; j < FN; ++j) acc[i][j] = MF::zero();
+    double dacc[FL ? FM : 1][FL ? FN : 1][FL ? MF::NR : 1];
+    if constexpr (FL) {
+#pragma unroll
+        for (int i = 0; i < FM; ++i)
+#pragma unroll
+            for (int j = 0; j < FN; ++j)
+#pragma unroll
+                for (int r = 0; r < MF::NR; ++r) dacc[i][j][r] = 0.0;
+    }
+    auto flush = [&]() {
+        if constexpr (FL) {
+#pragma unroll
+            for (int i = 0; i < FM; ++i)
+#pragma unroll
+                for (int j = 0; j < FN; ++j) {
+#pragma unroll
+                    for (int r = 0; r < MF::NR; ++r) dacc[i][j][r] += (double)acc[i][j][r];
+                    acc[i][j] = MF::zero();
+                }
+        }
+    };
 
     const int kslices = p.kp / BK;
     const int nsteps = kslices * out.npairs;
@@ -402,7 +522,7 @@ void i8_gemm_kernel(I8Args p) {
         store(0);
     }
     __syncthreads();
-    const int lr = lane & 15, lk = (lane >> 4) * 16;
+    const int lr = lane & (MS - 1), lk = (lane / MS) * 16;
     for (int s = 0; s < nsteps; ++s) {
         const int cur = s & 1;
         const bool more = s + 1 < nsteps;
@@ -410,24 +530,31 @@ void i8_gemm_kernel(I8Args p) {
         const int8_t* la = stage_a(cur);
         const int8_t* lb = stage_b(cur);
 #pragma unroll
-        for (int kk = 0; kk < BK; kk += 64) {
+        for (int kk = 0; kk < BK; kk += MF::KD) {
             v4i af[FM], bf[FN];
 #pragma unroll
-            for (int i = 0; i < FM; ++i) af[i] = *reinterpret_cast<const v4i*>(&la[(wm0 + i * 16 + lr) * RS + kk + lk]);
+            for (int i = 0; i < FM; ++i) af[i] = *reinterpret_cast<const v4i*>(&la[(wm0 + i * MS + lr) * RS + kk + lk]);
 #pragma unroll
-            for (int j = 0; j < FN; ++j) bf[j] = *reinterpret_cast<const v4i*>(&lb[(wn0 + j * 16 + lr) * RS + kk + lk]);
+            for (int j = 0; j < FN; ++j) bf[j] = *reinterpret_cast<const v4i*>(&lb[(wn0 + j * MS + lr) * RS + kk + lk]);
 #pragma unroll
             for (int i = 0; i < FM; ++i)
 #pragma unroll
-                for (int j = 0; j < FN; ++j)
-                    acc[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(af[i], bf[j], acc[i][j], 0, 0, 0);
+                for (int j = 0; j < FN; ++j) acc[i][j] = MF::mma(af[i], bf[j], acc[i][j]);
         }
         if (more) store(cur ^ 1);
+        if constexpr (FL) {
+            if ((s + 1) % kFlush == 0) flush();
+        }
         __syncthreads();
     }
+    flush();
+    // value of result item r of block (i, j) as an exact FP64 integer
+    auto value = [&](int i, int j, int r) -> double {
+        if constexpr (FL) return dacc[i][j][r];
+        else return (double)acc[i][j][r];
+    };
 
-    // epilogue: lane holds rows 4*(lane/16)+r of column lane%16 in each block
-    const int lq = (lane >> 4) * 4;
+    // epilogue: per block, result group g of this lane is rows group_row(g)+0..3 of column lr
     if (p.mode == 0) {
         const int m = out.modulus, y = out.y;
         const double inv_m = out.inv_m;
@@ -436,20 +563,25 @@ void i8_gemm_kernel(I8Args p) {
         for (int i = 0; i < FM; ++i)
 #pragma unroll
             for (int j = 0; j < FN; ++j) {
-                const int col = n0 + wn0 + j * 16 + lr, row = m0 + wm0 + i * 16 + lq;
-                if (col >= p.n || row >= p.m) continue;
-                uint32_t word = 0;
+                const int col = n0 + wn0 + j * MS + lr;
+                if (col >= p.n) continue;
 #pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    const double v = (double)y * (double)acc[i][j][r];  // |v| < 2^40, exact
-                    double q = floor(v * inv_m);
-                    double rr = fma(-q, (double)m, v);  // exact small integer
-                    if (rr >= m) rr -= m;
-                    if (rr < 0) rr += m;
-                    word |= ((uint32_t)(int)rr & 0xffu) << (8 * r);
+                for (int g = 0; g < NG; ++g) {
+                    const int row = m0 + wm0 + i * MS + MF::group_row(lane, g);
+                    if (row >= p.m) continue;
+                    uint32_t word = 0;
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const double v = (double)y * value(i, j, 4 * g + r);  // |v| < 2^40, exact
+                        double q = floor(v * inv_m);
+                        double rr = fma(-q, (double)m, v);  // exact small integer
+                        if (rr >= m) rr -= m;
+                        if (rr < 0) rr += m;
+                        word |= ((uint32_t)(int)rr & 0xffu) << (8 * r);
+                    }
+                    // rows beyond m land in the padded part of the plane (ldu >= m rounded to 4)
+                    *reinterpret_cast<uint32_t*>(u + (int64_t)col * p.ldu + row) = word;
                 }
-                // rows beyond m land in the padded part of the plane (ldu >= m rounded to 4)
-                *reinterpret_cast<uint32_t*>(u + (int64_t)col * p.ldu + row) = word;
             }
     } else {
         const double sc = out.scale;
@@ -457,14 +589,18 @@ void i8_gemm_kernel(I8Args p) {
         for (int i = 0; i < FM; ++i)
 #pragma unroll
             for (int j = 0; j < FN; ++j) {
-                const int col = n0 + wn0 + j * 16 + lr, row = m0 + wm0 + i * 16 + lq;
+                const int col = n0 + wn0 + j * MS + lr;
                 if (col >= p.n) continue;
-                double* w = p.w + (int64_t)col * p.ldw + row;
 #pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    if (row + r >= p.m) break;
-                    const double v = (double)acc[i][j][r] * sc;
-                    w[r] = p.accumulate ? w[r] + v : v;
+                for (int g = 0; g < NG; ++g) {
+                    const int row = m0 + wm0 + i * MS + MF::group_row(lane, g);
+                    double* w = p.w + (int64_t)col * p.ldw + row;
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        if (row + r >= p.m) break;
+                        const double v = value(i, j, 4 * g + r) * sc;
+                        w[r] = p.accumulate ? w[r] + v : v;
+                    }
                 }
             }
     }
@@ -583,11 +719,13 @@ hipError_t operand_stats(const double* x, int64_t ld, int rows, int k, bool kcon
 }
 
 hipError_t planes(const double* x, int64_t ld, int rows, int k, int kp, bool kcontig, const int* sigma,
-                  int8_t* out, int64_t plane_stride, int nplanes, bool crt, const CrtConsts* cc, hipStream_t s) {
+                  int8_t* out, int64_t plane_stride, int nplanes, bool crt, const CrtConsts* cc, hipStream_t s,
+                  bool bf16 = false) {
     PlaneArgs pa;
     memset(&pa, 0, sizeof(pa));
     pa.x = x; pa.ld = ld; pa.rows = rows; pa.k = k; pa.kp = kp; pa.sigma = sigma;
     pa.out = out; pa.plane_stride = plane_stride; pa.nplanes = nplanes; pa.crt = crt ? 1 : 0;
+    pa.bf16 = bf16 ? 1 : 0;
     if (crt)
         for (int l = 0; l < nplanes; ++l) { pa.m[l] = cc->m[l]; pa.inv_m[l] = cc->inv_m[l]; }
     dim3 grid((rows + 63) / 64, kp / 64);
@@ -597,13 +735,13 @@ hipError_t planes(const double* x, int64_t ld, int rows, int k, int kp, bool kco
 }
 }  // namespace
 
-template <int BM, int BN, int BK, int WM, int WN, int OCC>
+template <int BM, int BN, int BK, int WM, int WN, int OCC, class MF = I8Mfma<16>>
 static hipError_t i8_launch_cfg(I8Args p, int nout, hipStream_t s) {
     p.tiles_m = (p.m + BM - 1) / BM;
     p.tiles_n = (p.n + BN - 1) / BN;
     if (p.kp % BK != 0) return hipErrorInvalidValue;
     const size_t lds = 2 * (size_t)(BM + BN) * (BK + 16);
-    auto kern = i8_gemm_kernel<BM, BN, BK, WM, WN, OCC>;
+    auto kern = i8_gemm_kernel<BM, BN, BK, WM, WN, OCC, MF>;
     hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
                                        (int)lds);
     if (e != hipSuccess) return e;
@@ -621,30 +759,49 @@ hipError_t i8_gemm_launch(const I8Args& args, int nout, hipStream_t s, int tile)
         case 3: return i8_launch_cfg<256, 256, 64, 2, 2, 1>(args, nout, s);
         case 4: return i8_launch_cfg<256, 256, 128, 4, 2, 2>(args, nout, s);
         case 5: return i8_launch_cfg<256, 256, 64, 4, 2, 2>(args, nout, s);
+        case 6: return i8_launch_cfg<256, 256, 128, 4, 2, 2, I8Mfma<32>>(args, nout, s);
+        case 7: return i8_launch_cfg<256, 256, 64, 4, 2, 2, I8Mfma<32>>(args, nout, s);
+        case 8: return i8_launch_cfg<256, 128, 64, 2, 2, 2, I8Mfma<32>>(args, nout, s);
+        case 9: return i8_launch_cfg<128, 128, 128, 2, 2, 2, I8Mfma<32>>(args, nout, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+// BF16 tile configurations over bf16 planes (BK in bytes: 64 bf16 per step)
+hipError_t bf16_gemm_launch(const I8Args& args, int nout, hipStream_t s, int tile) {
+    switch (tile) {
+        case 0: return i8_launch_cfg<128, 128, 128, 4, 2, 2, Bf16Mfma<16>>(args, nout, s);
+        case 1: return i8_launch_cfg<128, 128, 128, 4, 2, 2, Bf16Mfma<32>>(args, nout, s);
+        case 2: return i8_launch_cfg<256, 128, 128, 4, 2, 1, Bf16Mfma<32>>(args, nout, s);
         default: return hipErrorInvalidValue;
     }
 }
 
 int i8_pick_tile(int m, int n, int kp) {
     const long long big = (long long)((m + 255) / 256) * ((n + 255) / 256);
-    if (kp % 128 == 0 && big >= 64) return 0;
+    if (kp % 128 == 0 && big >= 64) return 4;
     return 1;
 }
 
 int ozaki_dgemm(bool ta, bool tb, int m, int n, int k, double alpha, const double* a, int64_t lda, const double* b,
                 int64_t ldb, double beta, double* c, int64_t ldc, hipStream_t s, int mode, int mode_param, int tile) {
-    if (mode == ZYNUM_GPU_MODE_OZAKI_BF16) return ozaki_bf16_dgemm(ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, s, mode_param);
     const bool crt = (mode == ZYNUM_GPU_MODE_OZAKI_CRT);
+    const bool bf16 = (mode == ZYNUM_GPU_MODE_OZAKI_BF16);  // slices as bf16, FP32 MFMA accumulation
+    const int eb = bf16 ? 2 : 1;
     const int planes_n = crt ? (mode_param ? mode_param : 14) : (mode_param ? mode_param : 7);
     if (crt && (planes_n < 8 || planes_n > kMaxModuli)) return ZYNUM_GPU_DECLINE_UNSUPPORTED;
     if (!crt && (planes_n < 2 || planes_n > 8)) return ZYNUM_GPU_DECLINE_UNSUPPORTED;
-    if ((int64_t)k > (crt ? (1 << 17) : (1 << 18))) return ZYNUM_GPU_DECLINE_UNSUPPORTED;  // INT32 headroom
+    // INT32 headroom: CRT residues are in [-128, 127], so K products of at most 2^14 stay below 2^31 for
+    // K < 2^17; a slice level sums up to s pairs of products of at most 2^12, so s K < 2^19.  The BF16 form
+    // flushes its FP32 sums into FP64 and has no such limit (2^18 bounds its workspace).
+    const int64_t kmax = crt ? (1 << 17) - 1 : bf16 ? (1 << 18) : ((1 << 19) - 1) / planes_n;
+    if ((int64_t)k > kmax) return ZYNUM_GPU_DECLINE_UNSUPPORTED;
     CrtConsts cc;
     if (crt && !crt_setup(planes_n, k, cc)) return ZYNUM_GPU_DECLINE_UNSUPPORTED;
 
     const int kp = (k + 127) / 128 * 128;
     const int ldu = (m + 3) / 4 * 4;
-    const size_t plane_a = (size_t)m * kp, plane_b = (size_t)n * kp;
+    const size_t plane_a = (size_t)m * kp * eb, plane_b = (size_t)n * kp * eb;  // bytes
     size_t bytes = 0;
     bytes += 2 * ((size_t)m + (size_t)n) * 8 + 4096;             // max bits + sumsq
     bytes += ((size_t)m + (size_t)n) * 4 + 1024;                 // sigma
@@ -672,12 +829,12 @@ int ozaki_dgemm(bool ta, bool tb, int m, int n, int k, double alpha, const doubl
         row_exponent_kernel<<<(n + 255) / 256, 256, 0, s>>>(n, k, max_b, ss_b, cc.alpha, crt, sig_b);
         e = hipGetLastError();
     }
-    if (e == hipSuccess) e = planes(a, lda, m, k, kp, ta, sig_a, pa, plane_a, planes_n, crt, &cc, s);
-    if (e == hipSuccess) e = planes(b, ldb, n, k, kp, !tb, sig_b, pb, plane_b, planes_n, crt, &cc, s);
+    if (e == hipSuccess) e = planes(a, lda, m, k, kp, ta, sig_a, pa, plane_a, planes_n, crt, &cc, s, bf16);
+    if (e == hipSuccess) e = planes(b, ldb, n, k, kp, !tb, sig_b, pb, plane_b, planes_n, crt, &cc, s, bf16);
 
     I8Args ia;
     memset(&ia, 0, sizeof(ia));
-    ia.m = m; ia.n = n; ia.kp = kp;
+    ia.m = m; ia.n = n; ia.kp = kp * eb;
     ia.a = pa; ia.a_plane = plane_a; ia.b = pb; ia.b_plane = plane_b;
     if (crt && e == hipSuccess) {
         uint8_t* u = ws.take<uint8_t>((size_t)ldu * n * planes_n);
@@ -716,7 +873,8 @@ int ozaki_dgemm(bool ta, bool tb, int m, int n, int k, double alpha, const doubl
             lv.outs[0].npairs = np;
             lv.outs[0].scale = ldexp(1.0, -(12 + 7 * (t - 2)));
             lv.accumulate = (t == 2) ? 0 : 1;
-            e = i8_gemm_launch(lv, 1, s, tile >= 0 ? tile : i8_pick_tile(m, n, kp));
+            e = bf16 ? bf16_gemm_launch(lv, 1, s, tile >= 0 ? tile : 2)  // measured best at 8192^3
+                     : i8_gemm_launch(lv, 1, s, tile >= 0 ? tile : i8_pick_tile(m, n, kp));
         }
         if (e == hipSuccess) {
             dim3 grid((m + 255) / 256, n);
@@ -730,11 +888,4 @@ int ozaki_dgemm(bool ta, bool tb, int m, int n, int k, double alpha, const doubl
     return st;
 }
 
-}  // namespace zg
-
-namespace zg {
-int ozaki_bf16_dgemm(bool, bool, int, int, int, double, const double*, int64_t, const double*, int64_t, double,
-                     double*, int64_t, hipStream_t, int) {
-    return ZYNUM_GPU_DECLINE_UNSUPPORTED;
-}
 }  // namespace zg

@@ -40,7 +40,7 @@ struct I8Out {
     double inv_m;
 };
 
-// Batched INT8 GEMM over K-contiguous planes [rows][kp] (kp a multiple of 64).
+// Batched INT8 (or BF16) GEMM over K-contiguous planes [rows][kp], kp = bytes per row (a multiple of 128).
 // Output z (grid.z) sums the products of its plane pairs in INT32, then
 //   mode 0: stores u = (y * acc) mod m as one byte at u + z*u_plane + col*ldu + row
 //   mode 1: w[col*ldw + row] (+)= acc * scale
@@ -61,10 +61,14 @@ struct I8Args {
     I8Out outs[kMaxModuli];
 };
 
+// tile configurations 0..kI8Tiles-1 of i8_gemm_launch (0..5: 16x16x64 MFMA, 6..9: 32x32x32 MFMA)
+constexpr int kI8Tiles = 10;
 hipError_t i8_gemm_launch(const I8Args& args, int nout, hipStream_t s, int tile);
+// tile configurations 0..kBf16Tiles-1 of the BF16 slice GEMM (same I8Args, bf16 planes, kp in bytes)
+constexpr int kBf16Tiles = 3;
+hipError_t bf16_gemm_launch(const I8Args& args, int nout, hipStream_t s, int tile);
 int i8_pick_tile(int m, int n, int kp);
 
-int ozaki_bf16_dgemm(bool ta, bool tb, int m, int n, int k, double alpha, const double* a, int64_t lda,
-                     const double* b, int64_t ldb, double beta, double* c, int64_t ldc, hipStream_t s, int slices);
+
 
 }  // namespace zg
