@@ -181,8 +181,8 @@ def test_crt_integer_exact_shapes(torch_cuda, shape, ta, tb):
     assert np.array_equal(got, want), mismatch_report(got, want)
 
 
-I8_TILES = 10
-BF16_TILES = 3
+I8_TILES = 13
+BF16_TILES = 4
 
 
 @pytest.mark.parametrize("tile", range(1, I8_TILES + 1))

@@ -421,6 +421,66 @@ struct Bf16Mfma<32> {
     static __device__ __forceinline__ int group_row(int lane, int g) { return 8 * g + 4 * (lane >> 5); }
 };
 
+// Epilogue shared by the INT8/BF16 GEMM kernels: the wave's FM x FN blocks of
+// MS x MS results start at row r0, column c0; value(i, j, r) is result item r
+// of block (i, j) as an exact FP64 integer.  Per block, result group g of this
+// lane is rows group_row(g)+0..3 of column lane % MS.
+template <class MF, int FM, int FN, class V>
+__device__ __forceinline__ void i8_store(const I8Args& p, const I8Out& out, int r0, int c0, int lane, V value) {
+    constexpr int MS = MF::MSZ;
+    constexpr int NG = MF::NR / 4;
+    const int lr = lane & (MS - 1);
+    if (p.mode == 0) {
+        const int m = out.modulus, y = out.y;
+        const double inv_m = out.inv_m;
+        uint8_t* u = p.u + (int64_t)blockIdx.z * p.u_plane;
+#pragma unroll
+        for (int i = 0; i < FM; ++i)
+#pragma unroll
+            for (int j = 0; j < FN; ++j) {
+                const int col = c0 + j * MS + lr;
+                if (col >= p.n) continue;
+#pragma unroll
+                for (int g = 0; g < NG; ++g) {
+                    const int row = r0 + i * MS + MF::group_row(lane, g);
+                    if (row >= p.m) continue;
+                    uint32_t word = 0;
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const double v = (double)y * value(i, j, 4 * g + r);  // |v| < 2^40, exact
+                        double q = floor(v * inv_m);
+                        double rr = fma(-q, (double)m, v);  // exact small integer
+                        if (rr >= m) rr -= m;
+                        if (rr < 0) rr += m;
+                        word |= ((uint32_t)(int)rr & 0xffu) << (8 * r);
+                    }
+                    // rows beyond m land in the padded part of the plane (ldu >= m rounded to 4)
+                    *reinterpret_cast<uint32_t*>(u + (int64_t)col * p.ldu + row) = word;
+                }
+            }
+    } else {
+        const double sc = out.scale;
+#pragma unroll
+        for (int i = 0; i < FM; ++i)
+#pragma unroll
+            for (int j = 0; j < FN; ++j) {
+                const int col = c0 + j * MS + lr;
+                if (col >= p.n) continue;
+#pragma unroll
+                for (int g = 0; g < NG; ++g) {
+                    const int row = r0 + i * MS + MF::group_row(lane, g);
+                    double* w = p.w + (int64_t)col * p.ldw + row;
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        if (row + r >= p.m) break;
+                        const double v = value(i, j, 4 * g + r) * sc;
+                        w[r] = p.accumulate ? w[r] + v : v;
+                    }
+                }
+            }
+    }
+}
+
 // BK is in bytes (p.kp is the plane row length in bytes).  For the BF16 forms
 // every product of two slices is an integer of at most 2^12 in magnitude, so
 // FP32 sums of up to 4096 of them are exact; the FP32 accumulators are added
@@ -439,7 +499,6 @@ void i8_gemm_kernel(I8Args p) {
     constexpr int FM = WTM / MS, FN = WTN / MS;
     constexpr int LA = BM * CPR / NT;  // 16-byte loads per thread (A)
     constexpr int LB = BN * CPR / NT;
-    constexpr int NG = MF::NR / 4;     // groups of 4 consecutive result rows per lane
     static_assert(LA >= 1 && LB >= 1 && BM * CPR % NT == 0 && BN * CPR % NT == 0, "tile/workgroup mismatch");
     static_assert(BK % 64 == 0 && WTM % MS == 0 && WTN % MS == 0, "tile is a multiple of the MFMA shape");
 
@@ -554,56 +613,230 @@ void i8_gemm_kernel(I8Args p) {
         else return (double)acc[i][j][r];
     };
 
-    // epilogue: per block, result group g of this lane is rows group_row(g)+0..3 of column lr
-    if (p.mode == 0) {
-        const int m = out.modulus, y = out.y;
-        const double inv_m = out.inv_m;
-        uint8_t* u = p.u + (int64_t)blockIdx.z * p.u_plane;
-#pragma unroll
-        for (int i = 0; i < FM; ++i)
-#pragma unroll
-            for (int j = 0; j < FN; ++j) {
-                const int col = n0 + wn0 + j * MS + lr;
-                if (col >= p.n) continue;
-#pragma unroll
-                for (int g = 0; g < NG; ++g) {
-                    const int row = m0 + wm0 + i * MS + MF::group_row(lane, g);
-                    if (row >= p.m) continue;
-                    uint32_t word = 0;
-#pragma unroll
-                    for (int r = 0; r < 4; ++r) {
-                        const double v = (double)y * value(i, j, 4 * g + r);  // |v| < 2^40, exact
-                        double q = floor(v * inv_m);
-                        double rr = fma(-q, (double)m, v);  // exact small integer
-                        if (rr >= m) rr -= m;
-                        if (rr < 0) rr += m;
-                        word |= ((uint32_t)(int)rr & 0xffu) << (8 * r);
-                    }
-                    // rows beyond m land in the padded part of the plane (ldu >= m rounded to 4)
-                    *reinterpret_cast<uint32_t*>(u + (int64_t)col * p.ldu + row) = word;
-                }
-            }
-    } else {
-        const double sc = out.scale;
-#pragma unroll
-        for (int i = 0; i < FM; ++i)
-#pragma unroll
-            for (int j = 0; j < FN; ++j) {
-                const int col = n0 + wn0 + j * MS + lr;
-                if (col >= p.n) continue;
-#pragma unroll
-                for (int g = 0; g < NG; ++g) {
-                    const int row = m0 + wm0 + i * MS + MF::group_row(lane, g);
-                    double* w = p.w + (int64_t)col * p.ldw + row;
-#pragma unroll
-                    for (int r = 0; r < 4; ++r) {
-                        if (row + r >= p.m) break;
-                        const double v = value(i, j, 4 * g + r) * sc;
-                        w[r] = p.accumulate ? w[r] + v : v;
-                    }
-                }
-            }
+    i8_store<MF, FM, FN>(p, out, m0 + wm0, n0 + wn0, lane, value);
+}
+
+
+// ---------------------------------------------------------------------------
+// The same GEMM with global -> LDS direct loads (global_load_lds_dwordx4) in
+// an NS-stage ring, so NS-1 stages of 64 k-bytes are in flight while one is
+// multiplied and no registers stage the tiles.
+//
+// Stage layout: BM rows of A then BN rows of B, 64 bytes each.  A direct load
+// writes 64 lanes x 16 bytes = 16 consecutive rows; lane l fills slot l % 4 of
+// row l / 4 with chunk (slot ^ sw(row)) of that row, sw(row) = (row >> 3) & 3.
+// A 32x32 fragment read (32 lanes: 32 consecutive rows, one chunk) then hits
+// 32 distinct 16-byte bank groups of the 512-byte LDS row.
+//
+// Synchronisation per stage s: wait until this wave's loads of stage s have
+// landed (vmcnt), barrier (every wave's have, and every wave is done reading
+// stage s-1), refill stage s-1's buffer with stage s+NS-1, then read and
+// multiply stage s.  The LDS reads are issued as inline ds_read_b128 so that
+// the compiler does not wait for every load in flight before each of them;
+// their completion is waited for explicitly (lgkmcnt) before use.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void wait_vmcnt_max(int n) {
+    // s_waitcnt vmcnt(n), expcnt and lgkmcnt left alone (gfx9 encoding: vmcnt in bits 3:0 and 15:14)
+    switch (n) {
+        case 0: __builtin_amdgcn_s_waitcnt(0x0F70); break;
+        case 1: __builtin_amdgcn_s_waitcnt(0x0F71); break;
+        case 2: __builtin_amdgcn_s_waitcnt(0x0F72); break;
+        case 3: __builtin_amdgcn_s_waitcnt(0x0F73); break;
+        case 4: __builtin_amdgcn_s_waitcnt(0x0F74); break;
+        case 5: __builtin_amdgcn_s_waitcnt(0x0F75); break;
+        case 6: __builtin_amdgcn_s_waitcnt(0x0F76); break;
+        case 7: __builtin_amdgcn_s_waitcnt(0x0F77); break;
+        case 8: __builtin_amdgcn_s_waitcnt(0x0F78); break;
+        case 9: __builtin_amdgcn_s_waitcnt(0x0F79); break;
+        case 10: __builtin_amdgcn_s_waitcnt(0x0F7A); break;
+        case 11: __builtin_amdgcn_s_waitcnt(0x0F7B); break;
+        case 12: __builtin_amdgcn_s_waitcnt(0x0F7C); break;
+        case 13: __builtin_amdgcn_s_waitcnt(0x0F7D); break;
+        case 14: __builtin_amdgcn_s_waitcnt(0x0F7E); break;
+        case 15: __builtin_amdgcn_s_waitcnt(0x0F7F); break;
+        case 16: __builtin_amdgcn_s_waitcnt(0x4F70); break;
+        case 17: __builtin_amdgcn_s_waitcnt(0x4F71); break;
+        case 18: __builtin_amdgcn_s_waitcnt(0x4F72); break;
+        case 19: __builtin_amdgcn_s_waitcnt(0x4F73); break;
+        case 20: __builtin_amdgcn_s_waitcnt(0x4F74); break;
+        case 21: __builtin_amdgcn_s_waitcnt(0x4F75); break;
+        case 22: __builtin_amdgcn_s_waitcnt(0x4F76); break;
+        case 23: __builtin_amdgcn_s_waitcnt(0x4F77); break;
+        case 24: __builtin_amdgcn_s_waitcnt(0x4F78); break;
+        default: __builtin_amdgcn_s_waitcnt(0x0F70); break;
     }
+}
+
+__device__ __forceinline__ v4i lds_read16(uint32_t addr) {
+    v4i x;
+    asm volatile("ds_read_b128 %0, %1" : "=v"(x) : "v"(addr) : "memory");
+    return x;
+}
+
+template <int BM, int BN, int WM, int WN, int NS, int OCC, class MF>
+__global__ __launch_bounds__(64 * WM * WN) __attribute__((amdgpu_waves_per_eu(OCC)))
+void i8_dma_kernel(I8Args p) {
+    constexpr int MS = MF::MSZ;
+    static_assert(MS == 32, "the swizzle is laid out for 32-row fragment reads");
+    constexpr bool FL = MF::kFloat;
+    constexpr int BKS = 64;                   // k bytes per stage
+    constexpr int kFlush = 8192 / BKS;        // BF16: stages per exact FP32 run (BKS/2 products each)
+    typedef typename MF::acc_t acc_t;
+    constexpr int NW = WM * WN;
+    constexpr int WTM = BM / WM, WTN = BN / WN;
+    constexpr int FM = WTM / MS, FN = WTN / MS;
+    constexpr int SB = (BM + BN) * BKS;       // bytes per stage
+    constexpr int NQ = (BM + BN) / 16;        // direct loads (16 rows each) per stage
+    constexpr int PW = NQ / NW;               // per wave
+    static_assert(NQ % NW == 0 && (BM / 16) % NW == 0 && WTM % MS == 0 && WTN % MS == 0, "tile/workgroup mismatch");
+    static_assert(NS >= 3 && PW * (NS - 2) <= 24, "vmcnt immediates");
+
+    extern __shared__ __attribute__((aligned(16))) int8_t lds[];
+    const uint32_t lds0 = (uint32_t)(uintptr_t)(__attribute__((address_space(3))) int8_t*)lds;
+
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm0 = (wave % WM) * WTM, wn0 = (wave / WM) * WTN;
+    const int total = p.tiles_m * p.tiles_n;
+    int tm, tn;
+    grouped_tile(xcd_remap(blockIdx.x, total), p.tiles_m, p.tiles_n, 8, tm, tn);
+    const int m0 = tm * BM, n0 = tn * BN;
+    const I8Out& out = p.outs[blockIdx.z];
+
+    // per-lane source offsets of this wave's direct loads: load i of wave w is
+    // number q = w + NW i, A rows for i < PA, B rows after (rows past m / n
+    // read row m-1 / n-1: they only feed results that are never stored)
+    constexpr int PA = (BM / 16) / NW;
+    int64_t src[PW];
+#pragma unroll
+    for (int i = 0; i < PW; ++i) {
+        const int q = wave + NW * i;
+        const bool qa = i < PA;
+        const int rl = (qa ? q : q - BM / 16) * 16 + (lane >> 2);  // row within the A or B part
+        const int chunk = (lane & 3) ^ ((rl >> 3) & 3);
+        const int gr = qa ? min(m0 + rl, p.m - 1) : min(n0 + rl, p.n - 1);
+        src[i] = (int64_t)gr * p.kp + chunk * 16;
+    }
+
+    const int kslices = p.kp / BKS;
+    const int nsteps = kslices * out.npairs;
+    // next stage to issue: plane pair, k slice, ring buffer
+    int iss_pr = 0, iss_ks = 0, iss_buf = 0;
+    const int8_t* iss_a = p.a + (int64_t)out.pairs[0].a * p.a_plane;
+    const int8_t* iss_b = p.b + (int64_t)out.pairs[0].b * p.b_plane;
+    auto issue = [&]() {
+        int8_t* dst = lds + iss_buf * SB;
+        const int64_t ko = (int64_t)iss_ks * BKS;
+#pragma unroll
+        for (int i = 0; i < PW; ++i) {
+            const int q = wave + NW * i;
+            const int8_t* g = (i < PA ? iss_a : iss_b) + ko + src[i];
+            __builtin_amdgcn_global_load_lds((const void __attribute__((address_space(1)))*)g,
+                                             (void __attribute__((address_space(3)))*)(dst + q * 1024), 16, 0, 0);
+        }
+        iss_buf = (iss_buf + 1 == NS) ? 0 : iss_buf + 1;
+        if (++iss_ks == kslices) {
+            iss_ks = 0;
+            if (++iss_pr < out.npairs) {
+                iss_a = p.a + (int64_t)out.pairs[iss_pr].a * p.a_plane;
+                iss_b = p.b + (int64_t)out.pairs[iss_pr].b * p.b_plane;
+            }
+        }
+    };
+
+    acc_t acc[FM][FN];
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+        for (int j = 0; j < FN; ++j) acc[i][j] = MF::zero();
+    double dacc[FL ? FM : 1][FL ? FN : 1][FL ? MF::NR : 1];
+    if constexpr (FL) {
+#pragma unroll
+        for (int i = 0; i < FM; ++i)
+#pragma unroll
+            for (int j = 0; j < FN; ++j)
+#pragma unroll
+                for (int r = 0; r < MF::NR; ++r) dacc[i][j][r] = 0.0;
+    }
+
+    // fragment addresses within a stage (substep t adds chunk 2t)
+    const int lr = lane & 31, lc = lane >> 5;
+    uint32_t fa_off[FM], fb_off[FN];
+#pragma unroll
+    for (int i = 0; i < FM; ++i) {
+        const int r = wm0 + i * MS + lr;
+        fa_off[i] = (uint32_t)(r * BKS);
+    }
+#pragma unroll
+    for (int j = 0; j < FN; ++j) {
+        const int r = wn0 + j * MS + lr;
+        fb_off[j] = (uint32_t)(BM * BKS + r * BKS);
+    }
+    // swizzle of row r: (r >> 3) & 3; every fragment row above has the same low 5 bits as lr
+    const int sw = (lr >> 3) & 3;
+
+#pragma unroll
+    for (int s = 0; s < NS - 1; ++s)
+        if (s < nsteps) issue();
+
+    int rd_buf = 0;
+    for (int s = 0; s < nsteps; ++s) {
+        if (s + NS - 2 < nsteps) wait_vmcnt_max(PW * (NS - 2));
+        else wait_vmcnt_max(0);
+        __builtin_amdgcn_s_barrier();
+        if (s + NS - 1 < nsteps) issue();
+        const uint32_t base = lds0 + (uint32_t)(rd_buf * SB);
+        rd_buf = (rd_buf + 1 == NS) ? 0 : rd_buf + 1;
+        // both substeps' fragments, then one wait: the other wave on this SIMD
+        // keeps the MFMA pipe busy while these reads are in flight
+        v4i af[2][FM], bf[2][FN];
+#pragma unroll
+        for (int t = 0; t < 2; ++t) {
+            const uint32_t co = (uint32_t)((((2 * t + lc) ^ sw)) << 4);
+#pragma unroll
+            for (int i = 0; i < FM; ++i) af[t][i] = lds_read16(base + fa_off[i] + co);
+#pragma unroll
+            for (int j = 0; j < FN; ++j) bf[t][j] = lds_read16(base + fb_off[j] + co);
+        }
+        asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+#pragma unroll
+        for (int t = 0; t < 2; ++t) {
+#pragma unroll
+            for (int i = 0; i < FM; ++i) asm volatile("" : "+v"(af[t][i]));  // values are used after the wait
+#pragma unroll
+            for (int j = 0; j < FN; ++j) asm volatile("" : "+v"(bf[t][j]));
+        }
+#pragma unroll
+        for (int t = 0; t < 2; ++t)
+#pragma unroll
+            for (int i = 0; i < FM; ++i)
+#pragma unroll
+                for (int j = 0; j < FN; ++j) acc[i][j] = MF::mma(af[t][i], bf[t][j], acc[i][j]);
+        if constexpr (FL) {
+            if ((s + 1) % kFlush == 0) {
+#pragma unroll
+                for (int i = 0; i < FM; ++i)
+#pragma unroll
+                    for (int j = 0; j < FN; ++j) {
+#pragma unroll
+                        for (int r = 0; r < MF::NR; ++r) dacc[i][j][r] += (double)acc[i][j][r];
+                        acc[i][j] = MF::zero();
+                    }
+            }
+        }
+    }
+    if constexpr (FL) {
+#pragma unroll
+        for (int i = 0; i < FM; ++i)
+#pragma unroll
+            for (int j = 0; j < FN; ++j)
+#pragma unroll
+                for (int r = 0; r < MF::NR; ++r) dacc[i][j][r] += (double)acc[i][j][r];
+    }
+    auto value = [&](int i, int j, int r) -> double {
+        if constexpr (FL) return dacc[i][j][r];
+        else return (double)acc[i][j][r];
+    };
+    i8_store<MF, FM, FN>(p, out, m0 + wm0, n0 + wn0, lane, value);
 }
 
 // ---------------------------------------------------------------------------
@@ -750,6 +983,21 @@ static hipError_t i8_launch_cfg(I8Args p, int nout, hipStream_t s) {
     return hipGetLastError();
 }
 
+template <int BM, int BN, int WM, int WN, int NS, int OCC, class MF>
+static hipError_t i8_dma_launch_cfg(I8Args p, int nout, hipStream_t s) {
+    p.tiles_m = (p.m + BM - 1) / BM;
+    p.tiles_n = (p.n + BN - 1) / BN;
+    if (p.kp % 64 != 0 || p.m < 1 || p.n < 1) return hipErrorInvalidValue;
+    const size_t lds = (size_t)NS * (BM + BN) * 64;
+    auto kern = i8_dma_kernel<BM, BN, WM, WN, NS, OCC, MF>;
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)lds);
+    if (e != hipSuccess) return e;
+    dim3 grid(p.tiles_m * p.tiles_n, 1, nout);
+    kern<<<grid, 64 * WM * WN, lds, s>>>(p);
+    return hipGetLastError();
+}
+
 // INT8 tile configurations (BM x BN x BK, wave grid); kp must be a multiple of BK.
 hipError_t i8_gemm_launch(const I8Args& args, int nout, hipStream_t s, int tile) {
     switch (tile) {
@@ -763,6 +1011,10 @@ hipError_t i8_gemm_launch(const I8Args& args, int nout, hipStream_t s, int tile)
         case 7: return i8_launch_cfg<256, 256, 64, 4, 2, 2, I8Mfma<32>>(args, nout, s);
         case 8: return i8_launch_cfg<256, 128, 64, 2, 2, 2, I8Mfma<32>>(args, nout, s);
         case 9: return i8_launch_cfg<128, 128, 128, 2, 2, 2, I8Mfma<32>>(args, nout, s);
+        // direct-to-LDS ring (BK 64 per stage): stages, 32x32x32 MFMA
+        case 10: return i8_dma_launch_cfg<256, 256, 4, 2, 4, 2, I8Mfma<32>>(args, nout, s);
+        case 11: return i8_dma_launch_cfg<256, 256, 4, 2, 3, 2, I8Mfma<32>>(args, nout, s);
+        case 12: return i8_dma_launch_cfg<256, 128, 4, 2, 4, 2, I8Mfma<32>>(args, nout, s);
         default: return hipErrorInvalidValue;
     }
 }
@@ -773,6 +1025,7 @@ hipError_t bf16_gemm_launch(const I8Args& args, int nout, hipStream_t s, int til
         case 0: return i8_launch_cfg<128, 128, 128, 4, 2, 2, Bf16Mfma<16>>(args, nout, s);
         case 1: return i8_launch_cfg<128, 128, 128, 4, 2, 2, Bf16Mfma<32>>(args, nout, s);
         case 2: return i8_launch_cfg<256, 128, 128, 4, 2, 1, Bf16Mfma<32>>(args, nout, s);
+        case 3: return i8_dma_launch_cfg<256, 128, 4, 2, 4, 1, Bf16Mfma<32>>(args, nout, s);
         default: return hipErrorInvalidValue;
     }
 }
