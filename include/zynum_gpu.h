@@ -115,6 +115,9 @@ enum {
                                       by the Chinese remainder theorem (Ozaki scheme II) */
     ZYNUM_GPU_MODE_DD = 4          /* double-double FP64 VALU reference (accuracy oracle on the device) */
 };
+/* Split-precision modes: K < 2^17 (CRT) and slices * K < 2^19 (INT8 slices) keep the INT32 sums
+   exact; larger K declines with ZYNUM_GPU_DECLINE_UNSUPPORTED before writing.  A NaN or an
+   infinity in a row of op(A) or a column of op(B) makes that row / column of the result NaN. */
 
 /* request flags */
 enum {
@@ -140,7 +143,7 @@ typedef struct zynum_gpu_gemm_request {
     uint8_t mode_param;  /* slices or moduli count; 0 picks the default */
     uint8_t flags;       /* ZYNUM_GPU_FLAG_* */
     uint8_t tile;        /* tile configuration of the GEMM core: 0 = tuned choice, i + 1 = configuration i
-                            (plain: FP64/FP32 MFMA tiles 0..7; INT8 modes: 0..12; BF16: 0..3) */
+                            (plain: FP64/FP32 MFMA tiles 0..7; INT8 modes: 0..13; BF16: 0..3) */
     uint8_t reserved1[4];
 } zynum_gpu_gemm_request; /* 104 bytes */
 
@@ -160,7 +163,8 @@ typedef struct zynum_gpu_gemm_problem {
 
 /* Opaque grouped-GEMM plan: the problem list is validated, bucketed by size,
    assigned a tile shape per bucket by the autotuner table (tile = 0), or a
-   forced tile configuration (tile = i + 1: 32x32, 64x64), expanded
+   forced tile configuration (tile = i + 1: 0 small 32x64 / 64x32 tiles, 1 64x64,
+   2 128x128), expanded
    into output tiles in longest-first order and uploaded once.
    zynum_gpu_grouped_run issues one persistent launch per batch; it allocates
    nothing and may be captured into a HIP graph.  Problem pointers are

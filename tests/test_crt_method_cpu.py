@@ -85,3 +85,63 @@ def test_unnormalized_digits_lose_exactness():
     bad = sum(reconstruct(C, 14, cfg, normalize=False) != float(C)
               for C in (representable(rng, cfg[0]) for _ in range(2000)))
     assert bad > 0
+
+
+def fma_exact(a, b, c):
+    """fma(a, b, c) of doubles: the exact a*b + c rounded once (Fraction -> float rounds to nearest even)."""
+    from fractions import Fraction
+
+    return float(Fraction(a) * Fraction(b) + Fraction(c))
+
+
+def sym_mod(v, m, two_step=True):
+    """The residue extraction of the plane kernel, in IEEE doubles: one exact
+    reduction step, a second one when |r| > m, then the symmetric fix-up."""
+    inv_m = 1.0 / m
+    r = fma_exact(-float(round(v * inv_m)), float(m), v)
+    if two_step and abs(r) > m:
+        r = fma_exact(-float(round(r * inv_m)), float(m), r)
+    ri = int(r)
+    half = m >> 1
+    if ri >= half + (m & 1):
+        ri -= m
+    if ri < -half:
+        ri += m
+    return ri
+
+
+def exact_sym_mod(v, m):
+    r = int(v) % m
+    half = m >> 1
+    return r - m if r >= half + (m & 1) else r
+
+
+def test_residue_extraction_exact_for_every_modulus_count():
+    """Scaled inputs are integers with |a'| <= 2^(alpha+1); for every modulus
+    count the kernel's reduction returns the exact symmetric residue (the
+    largest alpha, 20 moduli, needs the second step)."""
+    rng = random.Random(11)
+    for nmod in (8, 14, 16, 20):
+        P = 1
+        for m in MODULI[:nmod]:
+            P *= m
+        alpha = (P.bit_length() - 2) // 2
+        for _ in range(400):
+            e = rng.randint(0, alpha + 1)
+            v = float(rng.randint(-2**52, 2**52)) * 2.0 ** max(0, e - 52)
+            v = float(round(v))
+            for m in MODULI[:nmod]:
+                assert sym_mod(v, m) == exact_sym_mod(v, m), (nmod, v, m)
+
+
+def test_single_reduction_step_is_not_enough_at_20_moduli():
+    """Planted mistake: dropping the conditional second step gives wrong
+    residues for the largest scaled inputs (|a'| near 2^79)."""
+    rng = random.Random(12)
+    bad = 0
+    for _ in range(200):
+        v = float(rng.randint(2**52, 2**53 - 1)) * 2.0**26
+        for m in MODULI[:20]:
+            if sym_mod(v, m, two_step=False) != exact_sym_mod(v, m):
+                bad += 1
+    assert bad > 0

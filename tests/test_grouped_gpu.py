@@ -30,7 +30,7 @@ def test_grouped_bitwise_equals_single_calls(torch_cuda, dtype_name, ta, tb):
     b.loop()
     torch.cuda.synchronize()
     want = b.C.clone()
-    for tile in (0, 1, 2):
+    for tile in (0, 1, 2, 3):
         b.reset_c()
         p = b.plan(tile)
         p.run()

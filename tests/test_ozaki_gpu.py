@@ -181,7 +181,7 @@ def test_crt_integer_exact_shapes(torch_cuda, shape, ta, tb):
     assert np.array_equal(got, want), mismatch_report(got, want)
 
 
-I8_TILES = 13
+I8_TILES = 14
 BF16_TILES = 4
 
 
@@ -257,3 +257,26 @@ def test_k_limits_of_integer_accumulation(torch_cuda):
     for mode, param in ((2, 8), (1, 7)):
         got = run(torch_cuda, "N", "N", m, n, k, 1.0, a, m, b, k, 0.0, np.zeros(m * n), m, mode, param)
         assert np.all(got == want), (mode, got[:3], want)
+
+
+@pytest.mark.parametrize("mode,param", [(3, 14), (1, 7), (2, 7)])
+def test_non_finite_rows_and_columns_give_nan(torch_cuda, mode, param):
+    """A NaN in a row of A or an infinity in a column of B makes that row /
+    column of the result NaN (FP64 would give NaN or +-inf there); every other
+    entry is unaffected and, on integer inputs, exact."""
+    m, n, k = 70, 50, 90
+    rng = np.random.default_rng(21)
+    A = rng.integers(-9, 10, (m, k)).astype(np.float64)
+    B = rng.integers(-9, 10, (k, n)).astype(np.float64)
+    A[3, 5] = np.nan
+    B[7, 11] = np.inf
+    c = np.zeros(m * n)
+    got = run(torch_cuda, "N", "N", m, n, k, 1.0, A.T.ravel(), m, B.T.ravel(), k, 0.0, c, m, mode, param).reshape(n, m).T
+    assert np.all(np.isnan(got[3, :])) and np.all(np.isnan(got[:, 11]))
+    mask = np.ones((m, n), bool)
+    mask[3, :] = False
+    mask[:, 11] = False
+    A[3, 5] = 0
+    B[7, 11] = 0
+    want = (A.astype(np.int64) @ B.astype(np.int64)).astype(np.float64)
+    assert np.array_equal(got[mask], want[mask])

@@ -5,13 +5,18 @@
 // validates the batch once (gemmError rules, fortran.zig:88-103), picks a tile
 // configuration per problem from the tuned size-bucket table
 // (grouped_tune.inc, written by zynum_amd/autotune.py), expands every problem
-// into output tiles, orders the tiles longest-first (LPT: tiles with the most
-// K work first) and uploads the list.  One persistent kernel (three workgroups
-// per CU) pulls tiles from an atomic counter, so small and large problems
-// share the machine without a launch per problem; every tile runs the same
-// MFMA tile routine as the plain GEMM, so each problem's result is bitwise the
-// result of a single plain GEMM call on it.  The launch pair (counter reset +
-// persistent kernel) allocates nothing and can be captured into a HIP graph.
+// into output tiles and orders them longest-first (LPT: tiles with the most K
+// work first).  Tiles are 128x128, 64x64 or 32x64 / 64x32 by size class, and a
+// remainder of at most half a tile is covered by the next smaller shape, so
+// little MFMA work is spent on padding.  Each
+// tile is uploaded as one self-contained descriptor (problem arguments, tile
+// origin, code path).  One persistent kernel (two 8-wave workgroups per CU) walks
+// the LPT list round-robin (workgroup w takes tiles w, w + G, w + 2G, ...),
+// loading the descriptor of its next tile while the current one runs, so no
+// atomic or dependent load sits between tiles.  Every tile runs the same MFMA
+// tile routine as the plain GEMM, so each problem's result is bitwise the
+// result of a single plain GEMM call on it.  The launch allocates nothing and
+// can be captured into a HIP graph.
 #include <string.h>
 
 #include <algorithm>
@@ -24,81 +29,74 @@
 
 namespace zg {
 
-template <typename T>
-struct GroupedProblem {
-    int m, n, k, ta, tb;
-    int64_t lda, ldb, ldc;
-    T alpha, beta;
-    const T* a;
-    const T* b;
-    T* c;
-};
-
-struct WorkItem {
-    int problem;
-    int path;  // cfg * 4 + ta * 2 + tb, or kScalePath
-    int m0, n0;
-};
-
 constexpr int kScalePath = 64;
-constexpr int kGroupedCfgs = 2;  // 0: 32x32, 1: 64x64 (2x2 waves, 256 threads)
 constexpr int kScaleTile = 64;
+constexpr int kGroupedThreads = 512;  // 8 waves: every tile shape below uses all of them
+// tuned configurations (size classes): 0 small (32x64 / 64x32 tiles), 1 64x64, 2 128x128
+constexpr int kGroupedCfgs = 3;
 
+// one output tile with everything its workgroup needs
 template <typename T>
-struct GroupedArgs {
-    const GroupedProblem<T>* problems;
-    const WorkItem* items;
-    int nitems;
-    int* counter;
+struct alignas(16) TileItem {
+    GemmArgs<T> p;  // the problem (tiles_m / tiles_n / group_m unused)
+    int m0, n0;
+    int path;       // shape * 4 + ta * 2 + tb, or kScalePath (k == 0 or alpha == 0: C = beta C)
+    int pad;
 };
 
-// K slices of 16 for both scalar types (the per-element MFMA order does not
-// depend on the slice depth)
-template <typename T>
-using GCfg0 = TileCfg<32, 32, 16, 2, 2, 3>;
-template <typename T>
-using GCfg1 = TileCfg<64, 64, 16, 2, 2, 3>;
+// Tile shapes (K slices of 16 for both scalar types; the per-element MFMA
+// order does not depend on the slice depth).  Shape ids: 0: 32x64, 1: 64x32,
+// 2: 64x64, 3: 128x128.
+using GShape0 = TileCfg<32, 64, 16, 2, 4, 2>;
+using GShape1 = TileCfg<64, 32, 16, 4, 2, 2>;
+using GShape2 = TileCfg<64, 64, 16, 4, 2, 2>;
+using GShape3 = TileCfg<128, 128, 16, 4, 2, 2>;
+constexpr int kShapeRows[4] = {32, 64, 64, 128};
+constexpr int kShapeCols[4] = {64, 32, 64, 128};
 
-__global__ void reset_counter_kernel(int* counter) { *counter = 0; }
-
 template <typename T>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
-void grouped_gemm_kernel(GroupedArgs<T> g) {
-    __shared__ __attribute__((aligned(16))) T lds_a[tile_lds_a<T, GCfg1<T>>()];
-    __shared__ __attribute__((aligned(16))) T lds_b[tile_lds_b<T, GCfg1<T>>()];
-    __shared__ int s_item;
-    for (;;) {
-        __syncthreads();  // the previous tile is done with LDS and s_item
-        if (threadIdx.x == 0) s_item = atomicAdd(g.counter, 1);
-        __syncthreads();
-        const int it = s_item;
-        if (it >= g.nitems) return;  // every wave of the workgroup sees the same item
-        const WorkItem w = g.items[it];
-        const GroupedProblem<T> pr = g.problems[w.problem];
-        GemmArgs<T> p;
-        p.m = pr.m; p.n = pr.n; p.k = pr.k;
-        p.lda = pr.lda; p.ldb = pr.ldb; p.ldc = pr.ldc;
-        p.alpha = pr.alpha; p.beta = pr.beta;
-        p.a = pr.a; p.b = pr.b; p.c = pr.c;
-        p.tiles_m = p.tiles_n = p.group_m = 1;
-        switch (w.path) {
-#define ZG_PATH(ID, CFG, TA, TB) \
-    case ID: gemm_tile<T, CFG<T>, TA, TB, 1>(p, w.m0, w.n0, lds_a, lds_b); break;
-            ZG_PATH(0, GCfg0, false, false) ZG_PATH(1, GCfg0, false, true)
-            ZG_PATH(2, GCfg0, true, false) ZG_PATH(3, GCfg0, true, true)
-            ZG_PATH(4, GCfg1, false, false) ZG_PATH(5, GCfg1, false, true)
-            ZG_PATH(6, GCfg1, true, false) ZG_PATH(7, GCfg1, true, true)
+__device__ __forceinline__ void run_item(const TileItem<T>& w, T* lds_a, T* lds_b) {
+    switch (w.path) {
+#define ZG_PATH(SH, CFG)                                                                              \
+    case SH * 4 + 0: gemm_tile<T, CFG, false, false, 1>(w.p, w.m0, w.n0, lds_a, lds_b); break;       \
+    case SH * 4 + 1: gemm_tile<T, CFG, false, true, 1>(w.p, w.m0, w.n0, lds_a, lds_b); break;        \
+    case SH * 4 + 2: gemm_tile<T, CFG, true, false, 1>(w.p, w.m0, w.n0, lds_a, lds_b); break;        \
+    case SH * 4 + 3: gemm_tile<T, CFG, true, true, 1>(w.p, w.m0, w.n0, lds_a, lds_b); break;
+        ZG_PATH(0, GShape0) ZG_PATH(1, GShape1) ZG_PATH(2, GShape2) ZG_PATH(3, GShape3)
 #undef ZG_PATH
-            default: {  // k == 0 or alpha == 0: C = beta * C on a 64 x 64 tile
-                for (int e = threadIdx.x; e < kScaleTile * kScaleTile; e += 256) {
-                    const int i = w.m0 + e % kScaleTile, j = w.n0 + e / kScaleTile;
-                    if (i < p.m && j < p.n) {
-                        T* cp = p.c + (int64_t)j * p.ldc + i;
-                        *cp = (p.beta == T(0)) ? T(0) : p.beta * *cp;
-                    }
+        default: {  // k == 0 or alpha == 0: C = beta * C on a 64 x 64 tile
+            for (int e = threadIdx.x; e < kScaleTile * kScaleTile; e += kGroupedThreads) {
+                const int i = w.m0 + e % kScaleTile, j = w.n0 + e / kScaleTile;
+                if (i < w.p.m && j < w.p.n) {
+                    T* cp = w.p.c + (int64_t)j * w.p.ldc + i;
+                    *cp = (w.p.beta == T(0)) ? T(0) : w.p.beta * *cp;
                 }
             }
         }
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kGroupedThreads) __attribute__((amdgpu_waves_per_eu(2)))
+void grouped_gemm_kernel(const TileItem<T>* __restrict__ items, int nitems) {
+    __shared__ __attribute__((aligned(16))) T lds_a[tile_lds_a<T, GShape3>()];
+    __shared__ __attribute__((aligned(16))) T lds_b[tile_lds_b<T, GShape3>()];
+    int it = blockIdx.x;
+    if (it >= nitems) return;
+    // descriptors are uniform: scalar loads, the next one issued before this
+    // tile's operand loads so it has landed by the tile's first LDS wait
+    TileItem<T> cur = items[it];
+    for (;;) {
+        const int nx = it + gridDim.x;
+        TileItem<T> nxt;
+        if (nx < nitems) nxt = items[nx];
+        run_item<T>(cur, lds_a, lds_b);
+        if (nx >= nitems) return;
+        // every wave is done reading this tile's LDS; its C stores may still be in flight
+        asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+        __builtin_amdgcn_s_barrier();
+        cur = nxt;
+        it = nx;
     }
 }
 
@@ -122,7 +120,7 @@ int grouped_pick(int m, int n, int k, int elem_bytes) {
     return 1;
 }
 
-static const int kCfgTile[kGroupedCfgs] = {32, 64};
+static const int kCfgTile[kGroupedCfgs] = {32, 64, 128};
 
 }  // namespace zg
 
@@ -135,9 +133,7 @@ struct zynum_gpu_grouped_plan {
     int nproblems = 0;
     int grid = 0;
     double flops = 0.0;
-    void* problems = nullptr;  // device GroupedProblem<T>[]
-    zg::WorkItem* items = nullptr;
-    int* counter = nullptr;
+    void* items = nullptr;  // device TileItem<T>[nitems], LPT order
     hipStream_t capture_stream = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -155,60 +151,103 @@ bool problem_valid(const zynum_gpu_gemm_problem& p) {
     return p.lda >= max1(nrowa) && p.ldb >= max1(nrowb) && p.ldc >= max1(p.m);
 }
 
+// Split [0, dim) into blocks of t, then (tuned plans) the remainder into
+// blocks of t/2 and t/4 down to 32, rounding the last one up: the covered
+// extent exceeds dim by less than 32 where whole t blocks could waste up to
+// t - 1.
+struct Seg {
+    int start, size, tile;
+};
+static std::vector<Seg> split_dim(int dim, int t, bool mixed) {
+    std::vector<Seg> out;
+    if (!mixed || t == 32) {
+        out.push_back({0, (dim + t - 1) / t * t, t});
+        return out;
+    }
+    int pos = 0;
+    for (int b = t; b >= 32 && pos < dim; b /= 2) {
+        int cnt = (dim - pos) / b;
+        const int rem = (dim - pos) - cnt * b;
+        // round up at this size when the rest would not fit the smaller sizes with less waste
+        if (b == 32 || rem > b - 32) cnt += rem > 0;
+        if (cnt > 0) {
+            out.push_back({pos, cnt * b, b});
+            pos += cnt * b;
+        }
+    }
+    return out;
+}
+
+// tile shape for blocks of rt rows x ct columns (rt, ct in {32, 64, 128})
+static int shape_for(int rt, int ct) {
+    if (rt == 128 && ct == 128) return 3;
+    if (rt >= 64 && ct >= 64) return 2;
+    if (rt >= 64) return 1;  // ct == 32
+    return 0;                // rt == 32
+}
+
 template <typename T>
 int build_plan(const zynum_gpu_gemm_problem* probs, int count, int force_cfg, zynum_gpu_grouped_plan* plan) {
-    std::vector<zg::GroupedProblem<T>> dp;
-    std::vector<zg::WorkItem> items;
+    std::vector<zg::TileItem<T>> items;
     std::vector<double> cost;
-    dp.reserve(count);
     for (int i = 0; i < count; ++i) {
         const zynum_gpu_gemm_problem& q = probs[i];
         if (!problem_valid(q)) return ZYNUM_GPU_DECLINE_ARGUMENT;
         if (q.m == 0 || q.n == 0) continue;
-        zg::GroupedProblem<T> g;
+        zg::TileItem<T> w;
+        memset(&w, 0, sizeof(w));
+        zg::GemmArgs<T>& g = w.p;
         g.m = q.m; g.n = q.n; g.k = q.k;
-        g.ta = q.transa != ZYNUM_GPU_NO_TRANS; g.tb = q.transb != ZYNUM_GPU_NO_TRANS;
         g.lda = q.lda; g.ldb = q.ldb; g.ldc = q.ldc;
         g.alpha = (T)q.alpha; g.beta = (T)q.beta;
         g.a = static_cast<const T*>(q.a); g.b = static_cast<const T*>(q.b); g.c = static_cast<T*>(q.c);
-        const int pi = (int)dp.size();
-        dp.push_back(g);
+        g.tiles_m = g.tiles_n = g.group_m = 1;
+        const int ta = q.transa != ZYNUM_GPU_NO_TRANS, tb = q.transb != ZYNUM_GPU_NO_TRANS;
+        auto add = [&](int path, int m0, int n0, double c) {
+            w.path = path; w.m0 = m0; w.n0 = n0;
+            items.push_back(w);
+            cost.push_back(c);
+        };
         if (q.k == 0 || g.alpha == T(0)) {
             for (int n0 = 0; n0 < q.n; n0 += zg::kScaleTile)
-                for (int m0 = 0; m0 < q.m; m0 += zg::kScaleTile) {
-                    items.push_back({pi, zg::kScalePath, m0, n0});
-                    cost.push_back(1.0 * zg::kScaleTile * zg::kScaleTile);
-                }
+                for (int m0 = 0; m0 < q.m; m0 += zg::kScaleTile)
+                    add(zg::kScalePath, m0, n0, 1.0 * zg::kScaleTile * zg::kScaleTile);
             continue;
         }
         plan->flops += 2.0 * q.m * q.n * (double)q.k;
         const int cfg = force_cfg >= 0 ? force_cfg : zg::grouped_pick(q.m, q.n, q.k, (int)sizeof(T));
         const int t = zg::kCfgTile[cfg];
-        const int path = cfg * 4 + g.ta * 2 + g.tb;
-        for (int n0 = 0; n0 < q.n; n0 += t)
-            for (int m0 = 0; m0 < q.m; m0 += t) {
-                items.push_back({pi, path, m0, n0});
-                cost.push_back((double)t * t * (double)q.k + 4096.0);
+        // the tuned choice covers a remainder of at most t/2 with half-size
+        // tiles; a forced configuration uses its size throughout
+        const bool mixed = force_cfg < 0;
+        // small class: 32-row blocks along the shorter side, 64 along the other
+        const int rt0 = (cfg == 0) ? (q.m <= q.n ? 32 : 64) : t;
+        const int ct0 = (cfg == 0) ? (q.m <= q.n ? 64 : 32) : t;
+        const std::vector<Seg> rs = split_dim(q.m, rt0, mixed && cfg > 0);
+        const std::vector<Seg> cs = split_dim(q.n, ct0, mixed && cfg > 0);
+        for (const Seg& c : cs)
+            for (const Seg& r : rs) {
+                const int sh = shape_for(r.tile, c.tile);
+                const int th = zg::kShapeRows[sh], tw = zg::kShapeCols[sh];
+                for (int n0 = c.start; n0 < c.start + c.size; n0 += tw)
+                    for (int m0 = r.start; m0 < r.start + r.size; m0 += th)
+                        add(sh * 4 + ta * 2 + tb, m0, n0, (double)th * tw * (double)q.k + 8192.0);
             }
     }
     // longest-processing-time order
     std::vector<int> order(items.size());
     for (size_t i = 0; i < order.size(); ++i) order[i] = (int)i;
     std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return cost[x] > cost[y]; });
-    std::vector<zg::WorkItem> sorted(items.size());
+    std::vector<zg::TileItem<T>> sorted(items.size());
     for (size_t i = 0; i < order.size(); ++i) sorted[i] = items[order[i]];
 
     plan->nitems = (int)sorted.size();
-    plan->nproblems = (int)dp.size();
-    const size_t pb = std::max<size_t>(1, dp.size()) * sizeof(zg::GroupedProblem<T>);
-    const size_t ib = std::max<size_t>(1, sorted.size()) * sizeof(zg::WorkItem);
-    if (hipMalloc(&plan->problems, pb) != hipSuccess || hipMalloc((void**)&plan->items, ib) != hipSuccess ||
-        hipMalloc((void**)&plan->counter, sizeof(int)) != hipSuccess) {
+    plan->nproblems = count;
+    const size_t ib = std::max<size_t>(1, sorted.size()) * sizeof(zg::TileItem<T>);
+    if (hipMalloc(&plan->items, ib) != hipSuccess) {
         (void)hipGetLastError();
         return ZYNUM_GPU_DECLINE_ALLOC;
     }
-    if (!dp.empty() && hipMemcpy(plan->problems, dp.data(), dp.size() * sizeof(dp[0]), hipMemcpyHostToDevice) != hipSuccess)
-        return ZYNUM_GPU_ERR_LAUNCH;
     if (!sorted.empty() &&
         hipMemcpy(plan->items, sorted.data(), sorted.size() * sizeof(sorted[0]), hipMemcpyHostToDevice) != hipSuccess)
         return ZYNUM_GPU_ERR_LAUNCH;
@@ -218,20 +257,22 @@ int build_plan(const zynum_gpu_gemm_problem* probs, int count, int force_cfg, zy
         if (hipGetDeviceProperties(&prop, dev) == hipSuccess) cus = prop.multiProcessorCount;
     }
     (void)hipGetLastError();
-    plan->grid = std::max(1, std::min(plan->nitems, 3 * cus));
+    // as many persistent workgroups as are resident at once (register and LDS
+    // limits of this kernel), so the round-robin share of every one is running
+    int per_cu = 1;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(zg::grouped_gemm_kernel<T>),
+                                                     zg::kGroupedThreads, 0) != hipSuccess || per_cu < 1)
+        per_cu = 1;
+    (void)hipGetLastError();
+    plan->grid = std::max(1, std::min(plan->nitems, per_cu * cus));
     return ZYNUM_GPU_OK;
 }
 
 template <typename T>
 hipError_t launch_plan(const zynum_gpu_grouped_plan* plan, hipStream_t s) {
-    zg::reset_counter_kernel<<<1, 1, 0, s>>>(plan->counter);
-    if (plan->nitems == 0) return hipGetLastError();
-    zg::GroupedArgs<T> g;
-    g.problems = static_cast<const zg::GroupedProblem<T>*>(plan->problems);
-    g.items = plan->items;
-    g.nitems = plan->nitems;
-    g.counter = plan->counter;
-    zg::grouped_gemm_kernel<T><<<plan->grid, 256, 0, s>>>(g);
+    if (plan->nitems == 0) return hipSuccess;
+    zg::grouped_gemm_kernel<T><<<plan->grid, zg::kGroupedThreads, 0, s>>>(static_cast<const zg::TileItem<T>*>(plan->items),
+                                                          plan->nitems);
     return hipGetLastError();
 }
 
@@ -299,10 +340,10 @@ void zynum_gpu_grouped_plan_destroy(zynum_gpu_grouped_plan* plan) {
     if (plan->exec) (void)hipGraphExecDestroy(plan->exec);
     if (plan->graph) (void)hipGraphDestroy(plan->graph);
     if (plan->capture_stream) (void)hipStreamDestroy(plan->capture_stream);
-    if (plan->problems || plan->items || plan->counter) (void)hipDeviceSynchronize();
-    if (plan->problems) (void)hipFree(plan->problems);
-    if (plan->items) (void)hipFree(plan->items);
-    if (plan->counter) (void)hipFree(plan->counter);
+    if (plan->items) {
+        (void)hipDeviceSynchronize();
+        (void)hipFree(plan->items);
+    }
     (void)hipGetLastError();
     delete plan;
 }

@@ -6,10 +6,10 @@
 // split the triangle in two halves, solve the first, update the second with
 // one GEMM (B2 -= op(A)21 X1, plain FP64 MFMA or split precision), recurse.
 // Leaves of at most 64 rows are solved in FP64 from LDS: the 64x64 triangle is
-// staged once per workgroup and each wave64 solves one vector (a column of B
-// for side L, a row for side R) with one lane per unknown, in the reference's
-// per-step order x_k /= a_kk; x_i = fma(a_ik, -x_k, x_i)
-// (matrix_vector/triangular.zig:375-385).
+// staged once per workgroup and each wave64 solves four vectors (columns of B
+// for side L, rows for side R) as interleaved chains with one lane per
+// unknown, in the reference's per-step order x_k /= a_kk;
+// x_i = fma(a_ik, -x_k, x_i) (matrix_vector/triangular.zig:375-385).
 //
 // SYRK (symmetric.zig:180-279, one strided dot per element) runs as one launch
 // over the lower (or upper) triangle of output tiles with the GEMM tile routine;
@@ -62,13 +62,19 @@ __device__ __forceinline__ T op_a(const LeafArgs<T>& p, int r, int c) {
     return p.trans ? p.a[(int64_t)r * p.lda + c] : p.a[(int64_t)c * p.lda + r];
 }
 
-// grid.x: groups of 64 vectors; 256 threads = 4 waves, 16 vectors per wave
+// grid.x: groups of kLeafVec vectors; 256 threads = 4 waves, each wave solves
+// kLeafVec / 4 vectors as independent interleaved chains (one lane per
+// unknown), so the per-step shuffle and FMA latencies of one vector overlap
+// with the others'.
+constexpr int kLeafVec = 16;
+constexpr int kChains = kLeafVec / 4;
+
 template <typename T>
 __global__ __launch_bounds__(256) void trsm_leaf_kernel(LeafArgs<T> p) {
-    __shared__ T M[kLeaf][kLeaf + 1];  // M[i][k]: coefficient of unknown k in equation i
-    __shared__ T V[kLeaf][kLeaf + 1];  // V[v][i]: vector v, unknown i
+    __shared__ T M[kLeaf][kLeaf + 1];     // M[i][k]: coefficient of unknown k in equation i
+    __shared__ T V[kLeafVec][kLeaf + 1];  // V[v][i]: vector v, unknown i
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int v0 = blockIdx.x * kLeaf;
+    const int v0 = blockIdx.x * kLeafVec;
     const int nb = p.nb;
     // Equation matrix: side L solves op(A) x = b  (M = op(A) block),
     // side R solves x op(A) = b, i.e. op(A)^T x^T = b^T (M = op(A)^T block).
@@ -88,37 +94,47 @@ __global__ __launch_bounds__(256) void trsm_leaf_kernel(LeafArgs<T> p) {
     }
     // vectors: side L, vector v = column v0 + v of B, unknown i = row i0 + i;
     // side R, vector v = row v0 + v, unknown i = column i0 + i
-    for (int e = tid; e < kLeaf * kLeaf; e += 256) {
+    for (int e = tid; e < kLeafVec * kLeaf; e += 256) {
         int v, i;
-        if (p.left) { i = e % kLeaf; v = e / kLeaf; } else { v = e % kLeaf; i = e / kLeaf; }
+        if (p.left) { i = e % kLeaf; v = e / kLeaf; } else { v = e % kLeafVec; i = e / kLeafVec; }
         T x = T(0);
         if (i < nb && v0 + v < p.nvec)
             x = p.left ? p.b[(int64_t)(v0 + v) * p.ldb + p.i0 + i] : p.b[(int64_t)(p.i0 + i) * p.ldb + v0 + v];
         V[v][i] = x;
     }
     __syncthreads();
-    for (int v = wave; v < kLeaf; v += 4) {
-        if (v0 + v >= p.nvec) break;
-        T x = V[v][lane];
-        if (m_lower) {
-            for (int k = 0; k < nb; ++k) {
-                if (lane == k && !p.unit) x = x / M[k][k];
-                const T xk = __shfl(x, k);
-                if (lane > k && lane < nb) x = fma_t(M[lane][k], -xk, x);
-            }
-        } else {
-            for (int k = nb - 1; k >= 0; --k) {
-                if (lane == k && !p.unit) x = x / M[k][k];
-                const T xk = __shfl(x, k);
-                if (lane < k) x = fma_t(M[lane][k], -xk, x);
+    T x[kChains];
+#pragma unroll
+    for (int c = 0; c < kChains; ++c) x[c] = V[wave * kChains + c][lane];
+    if (m_lower) {
+        for (int k = 0; k < nb; ++k) {
+            const T mk = M[lane][k];
+            const T dk = M[k][k];
+#pragma unroll
+            for (int c = 0; c < kChains; ++c) {
+                if (lane == k && !p.unit) x[c] = x[c] / dk;
+                const T xk = __shfl(x[c], k);
+                if (lane > k && lane < nb) x[c] = fma_t(mk, -xk, x[c]);
             }
         }
-        V[v][lane] = x;
+    } else {
+        for (int k = nb - 1; k >= 0; --k) {
+            const T mk = M[lane][k];
+            const T dk = M[k][k];
+#pragma unroll
+            for (int c = 0; c < kChains; ++c) {
+                if (lane == k && !p.unit) x[c] = x[c] / dk;
+                const T xk = __shfl(x[c], k);
+                if (lane < k) x[c] = fma_t(mk, -xk, x[c]);
+            }
+        }
     }
+#pragma unroll
+    for (int c = 0; c < kChains; ++c) V[wave * kChains + c][lane] = x[c];
     __syncthreads();
-    for (int e = tid; e < kLeaf * kLeaf; e += 256) {
+    for (int e = tid; e < kLeafVec * kLeaf; e += 256) {
         int v, i;
-        if (p.left) { i = e % kLeaf; v = e / kLeaf; } else { v = e % kLeaf; i = e / kLeaf; }
+        if (p.left) { i = e % kLeaf; v = e / kLeaf; } else { v = e % kLeafVec; i = e / kLeafVec; }
         if (i < nb && v0 + v < p.nvec) {
             if (p.left) p.b[(int64_t)(v0 + v) * p.ldb + p.i0 + i] = V[v][i];
             else p.b[(int64_t)(p.i0 + i) * p.ldb + v0 + v] = V[v][i];
@@ -171,7 +187,7 @@ int leaf(const TrsmCtx<T>& c, int i0, int nb) {
     p.left = c.left; p.lower_eff = c.lower_eff; p.trans = c.trans; p.unit = c.unit;
     p.i0 = i0; p.nb = nb;
     p.nvec = c.left ? c.n : c.m;
-    const int grid = (p.nvec + kLeaf - 1) / kLeaf;
+    const int grid = (p.nvec + kLeafVec - 1) / kLeafVec;
     trsm_leaf_kernel<T><<<grid, 256, 0, c.s>>>(p);
     return status_from_hip(hipGetLastError());
 }

@@ -148,73 +148,94 @@ __global__ void fill_u64_kernel(uint64_t* p, int64_t n, uint64_t v) {
 }
 
 // One pass over op(X): per-row max |x| (atomicMax on the bit pattern, which
-// orders like the value for non-negative doubles) and raw sum of squares.
-// Strided rows: one thread per row over a 256-deep k chunk.  Contiguous rows:
-// one wave per row, lanes over k.
+// orders like the value for non-negative doubles and puts NaN above +inf) and
+// the sum of squares of each 256-deep k chunk, stored per chunk so that the
+// row sum is formed in a fixed order (run-to-run reproducible scaling).
+// Strided rows: one thread per row and chunk.  Contiguous rows: one wave per
+// row, lanes over k.
 constexpr int kStatChunk = 256;
+// sigma of a row or column holding a NaN or an infinity: its results are NaN
+constexpr int kNonFinite = 0x7fffffff;
 
 template <bool KCONTIG>
 __global__ __launch_bounds__(256) void row_stats_kernel(const double* __restrict__ x, int64_t ld, int rows, int k,
-                                                         uint64_t* __restrict__ maxbits, double* __restrict__ sumsq) {
+                                                         uint64_t* __restrict__ maxbits, double* __restrict__ part) {
     const int k0 = blockIdx.y * kStatChunk, k1 = min(k, k0 + kStatChunk);
+    double* ps = part + (int64_t)blockIdx.y * rows;  // this chunk's partial sums
     if constexpr (!KCONTIG) {
         const int r = blockIdx.x * 256 + threadIdx.x;
         if (r >= rows) return;
         double mx = 0.0, ss = 0.0;
+        bool nan = false;  // fmax drops NaN operands: flag them separately
         const double* px = x + r + (int64_t)k0 * ld;
         int kk = k0;
         for (; kk + 4 <= k1; kk += 4, px += 4 * ld) {
             const double v0 = px[0], v1 = px[ld], v2 = px[2 * ld], v3 = px[3 * ld];
             mx = fmax(fmax(mx, fabs(v0)), fmax(fabs(v1), fmax(fabs(v2), fabs(v3))));
+            nan |= (v0 != v0) | (v1 != v1) | (v2 != v2) | (v3 != v3);
             ss = fma(v0, v0, ss); ss = fma(v1, v1, ss); ss = fma(v2, v2, ss); ss = fma(v3, v3, ss);
         }
         for (; kk < k1; ++kk, px += ld) {
             const double v = *px;
             mx = fmax(mx, fabs(v));
+            nan |= (v != v);
             ss = fma(v, v, ss);
         }
+        if (nan) mx = __longlong_as_double(0x7ff8000000000000ll);
         atomicMax((unsigned long long*)&maxbits[r], (unsigned long long)__double_as_longlong(mx));
-        atomicAdd(&sumsq[r], ss);
+        ps[r] = ss;
     } else {
         const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
         for (int rr = wave; rr < 64; rr += 4) {
             const int r = blockIdx.x * 64 + rr;
             if (r >= rows) break;
             double mx = 0.0, ss = 0.0;
+            bool nan = false;
             for (int kk = k0 + lane; kk < k1; kk += 64) {
                 const double v = x[kk + (int64_t)r * ld];
                 mx = fmax(mx, fabs(v));
+                nan |= (v != v);
                 ss = fma(v, v, ss);
             }
             for (int off = 32; off > 0; off >>= 1) {
                 mx = fmax(mx, __shfl_xor(mx, off));
+                nan |= (bool)__shfl_xor((int)nan, off);
                 ss += __shfl_xor(ss, off);
             }
+            if (nan) mx = __longlong_as_double(0x7ff8000000000000ll);
             if (lane == 0) {
                 atomicMax((unsigned long long*)&maxbits[r], (unsigned long long)__double_as_longlong(mx));
-                atomicAdd(&sumsq[r], ss);
+                ps[r] = ss;
             }
         }
     }
 }
 
 // Per-row scaling exponent sigma_r.
-//   CRT:    ||x_r 2^sigma||_2 <= 2^alpha.  The norm comes from the raw sum of
+//   CRT:    ||x_r 2^sigma||_2 <= 2^alpha.  The norm comes from the sum of
 //           squares when that is trustworthy (finite, and the max square is a
 //           normal number so no significant term underflowed); otherwise from
 //           the bound sqrt(k) max |x|.
 //   slices: max |x_r 2^sigma| < 1
+//   rows holding a NaN or an infinity: kNonFinite
 __global__ void row_exponent_kernel(int rows, int k, const uint64_t* __restrict__ maxbits,
-                                    const double* __restrict__ sumsq, int alpha, int crt, int* __restrict__ sigma) {
+                                    const double* __restrict__ part, int nchunks, int alpha, int crt,
+                                    int* __restrict__ sigma) {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= rows) return;
     const double mx = __longlong_as_double((long long)maxbits[r]);
-    if (mx == 0.0 || !isfinite(mx)) {
+    if (!isfinite(mx)) {
+        sigma[r] = kNonFinite;
+        return;
+    }
+    if (mx == 0.0) {
         sigma[r] = 0;
         return;
     }
     if (crt) {
-        const double m2 = mx * mx, ss = sumsq[r];
+        double ss = 0.0;
+        for (int c = 0; c < nchunks; ++c) ss += part[(int64_t)c * rows + r];
+        const double m2 = mx * mx;
         const bool ok = isfinite(ss) && isfinite(m2) && m2 >= 0x1p-1000 && ss >= m2;
         // relative rounding of a k-term sum is below k 2^-53 <= 2^-36 for k <= 2^17
         const double nrm = ok ? sqrt(ss) * (1.0 + 0x1p-30) : sqrt((double)k) * mx * (1.0 + 0x1p-30);
@@ -228,9 +249,10 @@ __global__ void row_exponent_kernel(int rows, int k, const uint64_t* __restrict_
 }
 
 // ---------------------------------------------------------------------------
-// plane extraction: op(X) (rows x k, FP64) -> nplanes int8 planes [rows][kp]
-// (K-contiguous, zero padded to kp).  64 x 64 tiles through LDS so both the
-// FP64 reads and the int8 writes are coalesced whatever the transpose.
+// plane extraction: op(X) (rows x k, FP64) -> nplanes int8 (or bf16) planes
+// [rows][kp] (K-contiguous, zero padded to kp).  32 x 128 tiles through LDS so
+// the FP64 reads are coalesced whatever the transpose and every plane row
+// segment written is 128 contiguous int8 (256 bf16) bytes.
 // ---------------------------------------------------------------------------
 struct PlaneArgs {
     const double* x;
@@ -248,15 +270,12 @@ struct PlaneArgs {
 
 __device__ __forceinline__ int sym_mod(double v, int m, double inv_m) {
     // v is an integer-valued double (|v| < 2^100); returns v mod m in [-m/2, m/2).
-    // Each step r <- r - m rint(r/m) is exact (fma of an exactly representable
-    // integer result); rint(r/m) is off by at most |r| 2^-52 / m, so the
-    // magnitude drops to below 2^(e - 44) per step and two steps reach |r| < 2m.
-    double r = v;
-#pragma unroll
-    for (int it = 0; it < 3; ++it) {
-        const double q = rint(r * inv_m);
-        r = fma(-q, (double)m, r);
-    }
+    // rint(v inv_m) differs from v/m by at most 1/2 + |v| 2^-52 / m, so
+    // r <- v - m rint(v inv_m) is an integer with |r| <= m/2 + |v| 2^-52 < 2^48
+    // and the fma computing it is exact.  When that leaves |r| > m (|v| above
+    // about 2^59) a second step leaves |r| <= m/2 + 1.
+    double r = fma(-rint(v * inv_m), (double)m, v);
+    if (fabs(r) > (double)m) r = fma(-rint(r * inv_m), (double)m, r);
     int ri = (int)r;
     const int half = m >> 1;
     if (ri >= half + (m & 1)) ri -= m;
@@ -266,42 +285,55 @@ __device__ __forceinline__ int sym_mod(double v, int m, double inv_m) {
 
 template <bool KCONTIG>
 __global__ __launch_bounds__(256) void plane_kernel(PlaneArgs p) {
-    __shared__ double tile[64][65];
-    const int r0 = blockIdx.x * 64, k0 = blockIdx.y * 64;
+    constexpr int TR = 32, TK = 128;
+    __shared__ double tile[TR][TK + 1];
+    const int r0 = blockIdx.x * TR, k0 = blockIdx.y * TK;
     const int t = threadIdx.x;
     // load: coalesced along the contiguous dimension
 #pragma unroll
-    for (int i = 0; i < 16; ++i) {
+    for (int i = 0; i < TR * TK / 256; ++i) {
         const int e = t + 256 * i;
-        const int c = e & 63, o = e >> 6;  // c along contiguous dim
-        const int r = KCONTIG ? o : c, kk = KCONTIG ? c : o;
+        int r, kk;
+        if constexpr (KCONTIG) { kk = e & (TK - 1); r = e / TK; }
+        else { r = e & (TR - 1); kk = e / TR; }
         const int gr = r0 + r, gk = k0 + kk;
         double v = 0.0;
         if (gr < p.rows && gk < p.k) v = KCONTIG ? p.x[gk + (int64_t)gr * p.ld] : p.x[gr + (int64_t)gk * p.ld];
         tile[r][kk] = v;
     }
     __syncthreads();
-    // each thread: one row, 16 consecutive k -> 16 bytes per plane
-    const int r = t >> 2, kq = (t & 3) * 16;
+    // each thread: one row, 16 consecutive k -> 16 (bf16: 32) bytes per plane
+    const int r = t >> 3, kq = (t & 7) * 16;
     const int gr = r0 + r;
     if (gr >= p.rows || k0 + kq >= p.kp) return;
     const int sg = p.sigma[gr];
     const int eb = p.bf16 ? 2 : 1;
     int8_t* dst = p.out + ((int64_t)gr * p.kp + k0 + kq) * eb;
+    if (sg == kNonFinite) {
+        // the row's results are set to NaN by the final kernel; its planes are zero
+        for (int l = 0; l < p.nplanes; ++l)
+            for (int h = 0; h < eb; ++h)
+                reinterpret_cast<int4*>(dst + (int64_t)l * p.plane_stride)[h] = int4{0, 0, 0, 0};
+        return;
+    }
     if (p.crt) {
+        double a[16];
+#pragma unroll
+        for (int b = 0; b < 16; ++b) a[b] = rint(ldexp(tile[r][kq + b], sg));  // exact integers, |a| <= 2^(alpha+1)
         for (int l = 0; l < p.nplanes; ++l) {
+            const int ml = p.m[l];
+            const double iml = p.inv_m[l];
             int4 packed;
-            int* pw = reinterpret_cast<int*>(&packed);
+            uint32_t* pw = reinterpret_cast<uint32_t*>(&packed);
 #pragma unroll
             for (int w = 0; w < 4; ++w) {
                 uint32_t word = 0;
 #pragma unroll
                 for (int b = 0; b < 4; ++b) {
-                    const double a = rint(ldexp(tile[r][kq + 4 * w + b], sg));
-                    const int res = sym_mod(a, p.m[l], p.inv_m[l]);
+                    const int res = sym_mod(a[4 * w + b], ml, iml);
                     word |= ((uint32_t)(uint8_t)(int8_t)res) << (8 * b);
                 }
-                pw[w] = (int)word;
+                pw[w] = word;
             }
             *reinterpret_cast<int4*>(dst + (int64_t)l * p.plane_stride) = packed;
         }
@@ -689,7 +721,7 @@ void i8_dma_kernel(I8Args p) {
     constexpr int NQ = (BM + BN) / 16;        // direct loads (16 rows each) per stage
     constexpr int PW = NQ / NW;               // per wave
     static_assert(NQ % NW == 0 && (BM / 16) % NW == 0 && WTM % MS == 0 && WTN % MS == 0, "tile/workgroup mismatch");
-    static_assert(NS >= 3 && PW * (NS - 2) <= 24, "vmcnt immediates");
+    static_assert(NS >= 3 && PW * (NS - 1) <= 24, "vmcnt immediates");
 
     extern __shared__ __attribute__((aligned(16))) int8_t lds[];
     const uint32_t lds0 = (uint32_t)(uintptr_t)(__attribute__((address_space(3))) int8_t*)lds;
@@ -774,43 +806,70 @@ void i8_dma_kernel(I8Args p) {
     // swizzle of row r: (r >> 3) & 3; every fragment row above has the same low 5 bits as lr
     const int sw = (lr >> 3) & 3;
 
-#pragma unroll
-    for (int s = 0; s < NS - 1; ++s)
-        if (s < nsteps) issue();
-
-    int rd_buf = 0;
-    for (int s = 0; s < nsteps; ++s) {
-        if (s + NS - 2 < nsteps) wait_vmcnt_max(PW * (NS - 2));
-        else wait_vmcnt_max(0);
-        __builtin_amdgcn_s_barrier();
-        if (s + NS - 1 < nsteps) issue();
-        const uint32_t base = lds0 + (uint32_t)(rd_buf * SB);
-        rd_buf = (rd_buf + 1 == NS) ? 0 : rd_buf + 1;
-        // both substeps' fragments, then one wait: the other wave on this SIMD
-        // keeps the MFMA pipe busy while these reads are in flight
-        v4i af[2][FM], bf[2][FN];
+    // Fragments of one stage (both 32-byte substeps).  The reads of stage s+1
+    // are issued before the MFMAs of stage s (sched_barrier keeps that order),
+    // so the LDS latency and bandwidth overlap the MFMA pipe.
+    struct Frag {
+        v4i a[2][FM];
+        v4i b[2][FN];
+    };
+    auto read_stage = [&](Frag& f, int buf) {
+        const uint32_t base = lds0 + (uint32_t)(buf * SB);
 #pragma unroll
         for (int t = 0; t < 2; ++t) {
             const uint32_t co = (uint32_t)((((2 * t + lc) ^ sw)) << 4);
 #pragma unroll
-            for (int i = 0; i < FM; ++i) af[t][i] = lds_read16(base + fa_off[i] + co);
+            for (int i = 0; i < FM; ++i) f.a[t][i] = lds_read16(base + fa_off[i] + co);
 #pragma unroll
-            for (int j = 0; j < FN; ++j) bf[t][j] = lds_read16(base + fb_off[j] + co);
+            for (int j = 0; j < FN; ++j) f.b[t][j] = lds_read16(base + fb_off[j] + co);
         }
-        asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+    };
+    auto tie = [&](Frag& f) {  // the values are used after the preceding lgkmcnt wait
 #pragma unroll
         for (int t = 0; t < 2; ++t) {
 #pragma unroll
-            for (int i = 0; i < FM; ++i) asm volatile("" : "+v"(af[t][i]));  // values are used after the wait
+            for (int i = 0; i < FM; ++i) asm volatile("" : "+v"(f.a[t][i]));
 #pragma unroll
-            for (int j = 0; j < FN; ++j) asm volatile("" : "+v"(bf[t][j]));
+            for (int j = 0; j < FN; ++j) asm volatile("" : "+v"(f.b[t][j]));
         }
+    };
+    auto multiply = [&](const Frag& f) {
 #pragma unroll
         for (int t = 0; t < 2; ++t)
 #pragma unroll
             for (int i = 0; i < FM; ++i)
 #pragma unroll
-                for (int j = 0; j < FN; ++j) acc[i][j] = MF::mma(af[t][i], bf[t][j], acc[i][j]);
+                for (int j = 0; j < FN; ++j) acc[i][j] = MF::mma(f.a[t][i], f.b[t][j], acc[i][j]);
+    };
+
+    // prologue: all NS buffers in flight, stage 0 read
+#pragma unroll
+    for (int s = 0; s < NS; ++s)
+        if (s < nsteps) issue();
+    Frag f0, f1;
+    if (nsteps > 0) {
+        if (nsteps >= NS) wait_vmcnt_max(PW * (NS - 1));
+        else wait_vmcnt_max(0);
+        __builtin_amdgcn_s_barrier();
+        read_stage(f0, 0);
+        asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+        tie(f0);
+    }
+    int rd_buf = (1 == NS) ? 0 : 1;  // buffer of stage s + 1
+    auto step = [&](Frag& cur, Frag& nxt, int s) {
+        // this wave's loads of stage s+1 have landed (outstanding: stages s+2 .. s+NS-1)
+        if (nsteps - 2 - s >= NS - 2) wait_vmcnt_max(PW * (NS - 2));
+        else wait_vmcnt_max(0);
+        // every wave's stage s+1 is in LDS and every wave has read stage s
+        __builtin_amdgcn_s_barrier();
+        if (s + NS < nsteps) issue();  // into stage s's buffer
+        if (s + 1 < nsteps) read_stage(nxt, rd_buf);
+        __builtin_amdgcn_sched_barrier(0);
+        multiply(cur);
+        __builtin_amdgcn_sched_barrier(0);
+        asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+        tie(nxt);
+        rd_buf = (rd_buf + 1 == NS) ? 0 : rd_buf + 1;
         if constexpr (FL) {
             if ((s + 1) % kFlush == 0) {
 #pragma unroll
@@ -823,7 +882,13 @@ void i8_dma_kernel(I8Args p) {
                     }
             }
         }
+    };
+    int s = 0;
+    for (; s + 1 < nsteps; s += 2) {
+        step(f0, f1, s);
+        step(f1, f0, s + 1);
     }
+    if (s < nsteps) step(f0, f1, s);
     if constexpr (FL) {
 #pragma unroll
         for (int i = 0; i < FM; ++i)
@@ -855,27 +920,37 @@ struct CrtArgs {
     CrtConsts cc;
 };
 
-// one thread per 4 consecutive rows of one column (one 32-bit load per plane)
+// One thread per 8 consecutive rows of one column: the residue bytes of all
+// moduli are loaded first (one 64-bit load per modulus, all in flight), then
+// summed into the CRT digits.  Rows m..ldu-1 of a residue plane are padding.
+constexpr int kCrtRows = 8;
 __global__ __launch_bounds__(256) void crt_reconstruct_kernel(CrtArgs p) {
-    const int i0 = (blockIdx.x * 256 + threadIdx.x) * 4;
+    const int i0 = (blockIdx.x * 256 + threadIdx.x) * kCrtRows;
     const int j = blockIdx.y;
     if (i0 >= p.m) return;
     const int nd = p.cc.ndig, nm = p.cc.nmod;
-    double S[4][kMaxDigits] = {};
     const uint8_t* u = p.u + (int64_t)j * p.ldu + i0;
-    for (int l = 0; l < nm; ++l) {
-        const uint32_t w = *reinterpret_cast<const uint32_t*>(u + (int64_t)l * p.u_plane);
+    uint2 w[kMaxModuli];
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const double ul = (double)((w >> (8 * r)) & 0xffu);
+    for (int l = 0; l < kMaxModuli; ++l)
+        if (l < nm) w[l] = *reinterpret_cast<const uint2*>(u + (int64_t)l * p.u_plane);
+    double S[kCrtRows][kMaxDigits] = {};
 #pragma unroll
-            for (int d = 0; d < kMaxDigits; ++d)
-                if (d < nd) S[r][d] = fma(p.cc.h[l][d], ul, S[r][d]);  // exact: integer < 2^53
+    for (int l = 0; l < kMaxModuli; ++l) {
+        if (l < nm) {
+#pragma unroll
+            for (int r = 0; r < kCrtRows; ++r) {
+                const uint32_t word = (r < 4) ? w[l].x : w[l].y;
+                const double ul = (double)((word >> (8 * (r & 3))) & 0xffu);
+#pragma unroll
+                for (int d = 0; d < kMaxDigits; ++d)
+                    if (d < nd) S[r][d] = fma(p.cc.h[l][d], ul, S[r][d]);  // exact: integer < 2^53
+            }
         }
     }
     const int sb = p.sig_b[j];
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
+    for (int r = 0; r < kCrtRows; ++r) {
         const int i = i0 + r;
         if (i >= p.m) break;
         double xq = 0.0;
@@ -902,7 +977,9 @@ __global__ __launch_bounds__(256) void crt_reconstruct_kernel(CrtArgs p) {
 #pragma unroll
         for (int d = kMaxDigits - 1; d >= 0; --d)
             if (d < nd) cp = ldexp(D[d], p.cc.dig_exp[d]) + cp;
-        const double prod = ldexp(cp, -(p.sig_a[i] + sb));
+        const int sa = p.sig_a[i];
+        const double prod = (sa == kNonFinite || sb == kNonFinite) ? __longlong_as_double(0x7ff8000000000000ll)
+                                                                   : ldexp(cp, -(sa + sb));
         double* cptr = p.c + (int64_t)j * p.ldc + i;
         *cptr = epilogue<double>(p.alpha, p.beta, prod, cptr);
     }
@@ -915,7 +992,9 @@ __global__ __launch_bounds__(256) void slices_finish_kernel(int m, int n, const 
                                                             double* __restrict__ c, int64_t ldc) {
     const int i = blockIdx.x * 256 + threadIdx.x, j = blockIdx.y;
     if (i >= m) return;
-    const double prod = ldexp(w[(int64_t)j * ldw + i], -(sig_a[i] + sig_b[j]));
+    const int sa = sig_a[i], sb = sig_b[j];
+    const double prod = (sa == kNonFinite || sb == kNonFinite) ? __longlong_as_double(0x7ff8000000000000ll)
+                                                               : ldexp(w[(int64_t)j * ldw + i], -(sa + sb));
     double* cptr = c + (int64_t)j * ldc + i;
     *cptr = epilogue<double>(alpha, beta, prod, cptr);
 }
@@ -937,16 +1016,15 @@ struct Workspace {
 };
 
 hipError_t operand_stats(const double* x, int64_t ld, int rows, int k, bool kcontig, uint64_t* maxbits,
-                         double* sumsq, hipStream_t s) {
+                         double* part, hipStream_t s) {
     const int fb = std::max(1, std::min(1024, (rows + 255) / 256));
     fill_u64_kernel<<<fb, 256, 0, s>>>(maxbits, rows, 0ull);
-    fill_u64_kernel<<<fb, 256, 0, s>>>(reinterpret_cast<uint64_t*>(sumsq), rows, 0ull);
     if (kcontig) {
         dim3 grid((rows + 63) / 64, (k + kStatChunk - 1) / kStatChunk);
-        row_stats_kernel<true><<<grid, 256, 0, s>>>(x, ld, rows, k, maxbits, sumsq);
+        row_stats_kernel<true><<<grid, 256, 0, s>>>(x, ld, rows, k, maxbits, part);
     } else {
         dim3 grid((rows + 255) / 256, (k + kStatChunk - 1) / kStatChunk);
-        row_stats_kernel<false><<<grid, 256, 0, s>>>(x, ld, rows, k, maxbits, sumsq);
+        row_stats_kernel<false><<<grid, 256, 0, s>>>(x, ld, rows, k, maxbits, part);
     }
     return hipGetLastError();
 }
@@ -961,7 +1039,7 @@ hipError_t planes(const double* x, int64_t ld, int rows, int k, int kp, bool kco
     pa.bf16 = bf16 ? 1 : 0;
     if (crt)
         for (int l = 0; l < nplanes; ++l) { pa.m[l] = cc->m[l]; pa.inv_m[l] = cc->inv_m[l]; }
-    dim3 grid((rows + 63) / 64, kp / 64);
+    dim3 grid((rows + 31) / 32, kp / 128);
     if (kcontig) plane_kernel<true><<<grid, 256, 0, s>>>(pa);
     else plane_kernel<false><<<grid, 256, 0, s>>>(pa);
     return hipGetLastError();
@@ -1015,6 +1093,7 @@ hipError_t i8_gemm_launch(const I8Args& args, int nout, hipStream_t s, int tile)
         case 10: return i8_dma_launch_cfg<256, 256, 4, 2, 4, 2, I8Mfma<32>>(args, nout, s);
         case 11: return i8_dma_launch_cfg<256, 256, 4, 2, 3, 2, I8Mfma<32>>(args, nout, s);
         case 12: return i8_dma_launch_cfg<256, 128, 4, 2, 4, 2, I8Mfma<32>>(args, nout, s);
+        case 13: return i8_dma_launch_cfg<256, 256, 2, 2, 4, 1, I8Mfma<32>>(args, nout, s);
         default: return hipErrorInvalidValue;
     }
 }
@@ -1030,9 +1109,20 @@ hipError_t bf16_gemm_launch(const I8Args& args, int nout, hipStream_t s, int til
     }
 }
 
+namespace {
+struct I8TuneRow {
+    long long min_tiles256;
+    int cfg;
+};
+#include "i8_tune.inc"
+}  // namespace
+
+// kp is a multiple of 128, which every configuration accepts
 int i8_pick_tile(int m, int n, int kp) {
-    const long long big = (long long)((m + 255) / 256) * ((n + 255) / 256);
-    if (kp % 128 == 0 && big >= 64) return 4;
+    (void)kp;
+    const long long tiles = (long long)((m + 255) / 256) * ((n + 255) / 256);
+    for (const I8TuneRow& r : kI8TuneTable)
+        if (tiles >= r.min_tiles256) return r.cfg;
     return 1;
 }
 
@@ -1053,10 +1143,11 @@ int ozaki_dgemm(bool ta, bool tb, int m, int n, int k, double alpha, const doubl
     if (crt && !crt_setup(planes_n, k, cc)) return ZYNUM_GPU_DECLINE_UNSUPPORTED;
 
     const int kp = (k + 127) / 128 * 128;
-    const int ldu = (m + 3) / 4 * 4;
+    const int ldu = (m + kCrtRows - 1) / kCrtRows * kCrtRows;  // residue plane rows (the GEMM writes 4 at a time)
     const size_t plane_a = (size_t)m * kp * eb, plane_b = (size_t)n * kp * eb;  // bytes
     size_t bytes = 0;
-    bytes += 2 * ((size_t)m + (size_t)n) * 8 + 4096;             // max bits + sumsq
+    const int nchunks = (k + kStatChunk - 1) / kStatChunk;
+    bytes += ((size_t)m + (size_t)n) * 8 * (1 + (size_t)nchunks) + 4096;  // max bits + chunk sums of squares
     bytes += ((size_t)m + (size_t)n) * 4 + 1024;                 // sigma
     bytes += (plane_a + plane_b) * planes_n + 1024;              // planes
     bytes += crt ? (size_t)ldu * n * planes_n + 256 : (size_t)m * n * 8 + 256;  // u bytes / FP64 level sums
@@ -1066,8 +1157,8 @@ int ozaki_dgemm(bool ta, bool tb, int m, int n, int k, double alpha, const doubl
     Workspace ws{static_cast<char*>(wsp), 0};
     uint64_t* max_a = ws.take<uint64_t>(m);
     uint64_t* max_b = ws.take<uint64_t>(n);
-    double* ss_a = ws.take<double>(m);
-    double* ss_b = ws.take<double>(n);
+    double* ss_a = ws.take<double>((size_t)m * nchunks);
+    double* ss_b = ws.take<double>((size_t)n * nchunks);
     int* sig_a = ws.take<int>(m);
     int* sig_b = ws.take<int>(n);
     int8_t* pa = ws.take<int8_t>(plane_a * planes_n);
@@ -1078,8 +1169,8 @@ int ozaki_dgemm(bool ta, bool tb, int m, int n, int k, double alpha, const doubl
     e = operand_stats(a, lda, m, k, ta, max_a, ss_a, s);
     if (e == hipSuccess) e = operand_stats(b, ldb, n, k, !tb, max_b, ss_b, s);
     if (e == hipSuccess) {
-        row_exponent_kernel<<<(m + 255) / 256, 256, 0, s>>>(m, k, max_a, ss_a, cc.alpha, crt, sig_a);
-        row_exponent_kernel<<<(n + 255) / 256, 256, 0, s>>>(n, k, max_b, ss_b, cc.alpha, crt, sig_b);
+        row_exponent_kernel<<<(m + 255) / 256, 256, 0, s>>>(m, k, max_a, ss_a, nchunks, cc.alpha, crt, sig_a);
+        row_exponent_kernel<<<(n + 255) / 256, 256, 0, s>>>(n, k, max_b, ss_b, nchunks, cc.alpha, crt, sig_b);
         e = hipGetLastError();
     }
     if (e == hipSuccess) e = planes(a, lda, m, k, kp, ta, sig_a, pa, plane_a, planes_n, crt, &cc, s, bf16);
@@ -1106,7 +1197,7 @@ int ozaki_dgemm(bool ta, bool tb, int m, int n, int k, double alpha, const doubl
             ca.m = m; ca.n = n; ca.u = u; ca.u_plane = ia.u_plane; ca.ldu = ldu;
             ca.sig_a = sig_a; ca.sig_b = sig_b; ca.alpha = alpha; ca.beta = beta; ca.c = c; ca.ldc = ldc;
             ca.cc = cc;
-            dim3 grid((m + 1023) / 1024, n);
+            dim3 grid((m + 256 * kCrtRows - 1) / (256 * kCrtRows), n);
             crt_reconstruct_kernel<<<grid, 256, 0, s>>>(ca);
             e = hipGetLastError();
         }
