@@ -96,11 +96,18 @@ struct TileCfg {
     static constexpr int NT = 64 * WM_ * WN_;
 };
 
+// Padding (bytes) of each k-row of an LDS operand panel; a build knob for tuning.
+#ifndef ZG_LDS_PAD_BYTES
+#define ZG_LDS_PAD_BYTES 16
+#endif
+template <typename T>
+constexpr int lds_pad() { return ZG_LDS_PAD_BYTES / (int)sizeof(T); }
+
 // LDS elements one tile configuration needs for its two operand stages.
 template <typename T, class CF>
-constexpr int tile_lds_a() { return 2 * CF::BK * (CF::BM + 16 / (int)sizeof(T)); }
+constexpr int tile_lds_a() { return 2 * CF::BK * (CF::BM + lds_pad<T>()); }
 template <typename T, class CF>
-constexpr int tile_lds_b() { return 2 * CF::BK * (CF::BN + 16 / (int)sizeof(T)); }
+constexpr int tile_lds_b() { return 2 * CF::BK * (CF::BN + lds_pad<T>()); }
 
 // C[m0:m0+BM, n0:n0+BN] = alpha * op(A) op(B) + beta * C for one output tile,
 // run by all CF::NT threads of the workgroup.  The K loop, the per-element
@@ -114,7 +121,7 @@ template <typename T, class CF, bool TA, bool TB, int VEC, int TRI = 0>
 __device__ __forceinline__ void gemm_tile(const GemmArgs<T>& p, int m0, int n0, T* __restrict__ lds_a,
                                           T* __restrict__ lds_b) {
     constexpr int BM = CF::BM, BN = CF::BN, BK = CF::BK, NT = CF::NT;
-    constexpr int PAD = 16 / sizeof(T);
+    constexpr int PAD = lds_pad<T>();
     constexpr int WTM = BM / CF::WM, WTN = BN / CF::WN;
     constexpr int FM = WTM / 16, FN = WTN / 16;
     static_assert(FM >= 1 && FN >= 1, "wave tile smaller than one MFMA block");

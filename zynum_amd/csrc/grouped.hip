@@ -10,10 +10,10 @@
 // remainder of at most half a tile is covered by the next smaller shape, so
 // little MFMA work is spent on padding.  Each
 // tile is uploaded as one self-contained descriptor (problem arguments, tile
-// origin, code path).  One persistent kernel (two 8-wave workgroups per CU) walks
-// the LPT list round-robin (workgroup w takes tiles w, w + G, w + 2G, ...),
-// loading the descriptor of its next tile while the current one runs, so no
-// atomic or dependent load sits between tiles.  Every tile runs the same MFMA
+// origin, code path).  One persistent kernel per tile path walks that path's
+// LPT list round-robin (workgroup w takes tiles w, w + G, w + 2G, ...),
+// loading the descriptor and first K slice of its next tile while the current
+// one finishes, so no atomic or dependent load sits between tiles.  Every tile runs the same MFMA
 // tile routine as the plain GEMM, so each problem's result is bitwise the
 // result of a single plain GEMM call on it.  The launch allocates nothing and
 // can be captured into a HIP graph.
@@ -31,7 +31,6 @@ namespace zg {
 
 constexpr int kScalePath = 64;
 constexpr int kScaleTile = 64;
-constexpr int kGroupedThreads = 512;  // 8 waves: every tile shape below uses all of them
 // tuned configurations (size classes): 0 small (32x64 / 64x32 tiles), 1 64x64, 2 128x128
 constexpr int kGroupedCfgs = 3;
 
@@ -40,65 +39,167 @@ template <typename T>
 struct alignas(16) TileItem {
     GemmArgs<T> p;  // the problem (tiles_m / tiles_n / group_m unused)
     int m0, n0;
-    int path;       // shape * 4 + ta * 2 + tb, or kScalePath (k == 0 or alpha == 0: C = beta C)
+    int path;       // shape * 8 + vec * 4 + ta * 2 + tb, or kScalePath (k == 0 or alpha == 0: C = beta C)
     int pad;
 };
 
 // Tile shapes (K slices of 16 for both scalar types; the per-element MFMA
 // order does not depend on the slice depth).  Shape ids: 0: 32x64, 1: 64x32,
 // 2: 64x64, 3: 128x128.
-using GShape0 = TileCfg<32, 64, 16, 2, 4, 2>;
-using GShape1 = TileCfg<64, 32, 16, 4, 2, 2>;
-using GShape2 = TileCfg<64, 64, 16, 4, 2, 2>;
+// The small shapes run four waves in a 2x2 grid, so each wave issues a whole
+// (BM/2) x (BN/2) block of MFMAs per K slice: enough work per barrier to cover
+// the next slice's load latency with the short K of grouped problems.  The
+// 128x128 shape runs eight waves (4x2), the plain GEMM's best layout.
+// OCC: waves per SIMD the registers are sized for.
+using GShape0 = TileCfg<32, 64, 16, 2, 2, 4>;
+using GShape1 = TileCfg<64, 32, 16, 2, 2, 4>;
+using GShape2 = TileCfg<64, 64, 16, 2, 2, 4>;
 using GShape3 = TileCfg<128, 128, 16, 4, 2, 2>;
 constexpr int kShapeRows[4] = {32, 64, 64, 128};
 constexpr int kShapeCols[4] = {64, 32, 64, 128};
 
-template <typename T>
-__device__ __forceinline__ void run_item(const TileItem<T>& w, T* lds_a, T* lds_b) {
-    switch (w.path) {
-#define ZG_PATH(SH, CFG)                                                                              \
-    case SH * 4 + 0: gemm_tile<T, CFG, false, false, 1>(w.p, w.m0, w.n0, lds_a, lds_b); break;       \
-    case SH * 4 + 1: gemm_tile<T, CFG, false, true, 1>(w.p, w.m0, w.n0, lds_a, lds_b); break;        \
-    case SH * 4 + 2: gemm_tile<T, CFG, true, false, 1>(w.p, w.m0, w.n0, lds_a, lds_b); break;        \
-    case SH * 4 + 3: gemm_tile<T, CFG, true, true, 1>(w.p, w.m0, w.n0, lds_a, lds_b); break;
-        ZG_PATH(0, GShape0) ZG_PATH(1, GShape1) ZG_PATH(2, GShape2) ZG_PATH(3, GShape3)
-#undef ZG_PATH
-        default: {  // k == 0 or alpha == 0: C = beta * C on a 64 x 64 tile
-            for (int e = threadIdx.x; e < kScaleTile * kScaleTile; e += kGroupedThreads) {
-                const int i = w.m0 + e % kScaleTile, j = w.n0 + e / kScaleTile;
-                if (i < w.p.m && j < w.p.n) {
-                    T* cp = w.p.c + (int64_t)j * w.p.ldc + i;
-                    *cp = (w.p.beta == T(0)) ? T(0) : w.p.beta * *cp;
+// One persistent kernel per tile path (shape and transposes), so each runs
+// with only its own registers and LDS.  A workgroup walks its share of the
+// path's LPT list round-robin.  Tiles are software-pipelined across tile
+// boundaries: the last K step of a tile already loads the first K slice of the
+// workgroup's next tile, so that tile's operand latency overlaps this one's
+// last MFMAs and epilogue.  The per-element arithmetic (16x16x4 MFMA steps in
+// increasing k, then the alpha/beta epilogue) is the plain GEMM tile's, so
+// results are bitwise those of single GEMM calls.
+template <typename T, class CF, bool TA, bool TB, int VEC>
+__global__ __launch_bounds__(CF::NT) __attribute__((amdgpu_waves_per_eu(CF::OCC))) void grouped_path_kernel(const TileItem<T>* __restrict__ items, int nitems) {
+    constexpr int BM = CF::BM, BN = CF::BN, BK = CF::BK, NT = CF::NT;
+    constexpr int PAD = lds_pad<T>();
+    constexpr int WTM = BM / CF::WM, WTN = BN / CF::WN;
+    constexpr int FM = WTM / 16, FN = WTN / 16;
+    static_assert(FM >= 1 && FN >= 1, "wave tile smaller than one MFMA block");
+    using M = Mfma<T>;
+    using Acc = typename M::acc_t;
+    __shared__ __attribute__((aligned(16))) T lds_a[tile_lds_a<T, CF>()];
+    __shared__ __attribute__((aligned(16))) T lds_b[tile_lds_b<T, CF>()];
+
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm0 = (wave % CF::WM) * WTM, wn0 = (wave / CF::WM) * WTN;
+    const int lr = lane & 15, lk = lane >> 4;
+    const int ar = M::a_row(lr);
+
+    int it = blockIdx.x;
+    if (it >= nitems) return;
+    // vector width per operand: at most the elements each thread stages
+    constexpr int VA = (VEC < BM * BK / NT) ? VEC : BM * BK / NT;
+    constexpr int VB = (VEC < BN * BK / NT) ? VEC : BN * BK / NT;
+    Stage<T, BM, BK, TA, VA, NT, PAD> sa;
+    Stage<T, BN, BK, !TB, VB, NT, PAD> sb;
+    auto load_slice = [&](const TileItem<T>& w, int k0) {
+        const GemmArgs<T>& p = w.p;
+        const bool full = (w.m0 + BM <= p.m) && (w.n0 + BN <= p.n) && (k0 + BK <= p.k);
+        sa.load(p.a, p.lda, w.m0, k0, p.m, p.k, full, tid);
+        sb.load(p.b, p.ldb, w.n0, k0, p.n, p.k, full, tid);
+    };
+    auto barrier = [&]() {  // LDS reads and writes of this step done; global loads may stay in flight
+        asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+        __builtin_amdgcn_s_barrier();
+    };
+
+    // Descriptors are uniform (scalar loads); the next one is read only where
+    // it is used, so it does not hold registers through the tile.
+    TileItem<T> cur = items[it];
+    load_slice(cur, 0);
+    for (;;) {
+        const int nx = it + gridDim.x;
+        const bool has_next = nx < nitems;
+        const GemmArgs<T>& p = cur.p;
+        const int m0 = cur.m0, n0 = cur.n0;
+        const int nk = (p.k + BK - 1) / BK;
+        sa.store(lds_a, tid);
+        sb.store(lds_b, tid);
+        barrier();
+        Acc acc[FM][FN];
+#pragma unroll
+        for (int i = 0; i < FM; ++i)
+#pragma unroll
+            for (int j = 0; j < FN; ++j) acc[i][j] = M::zero();
+        for (int t = 0; t < nk; ++t) {
+            const int cb = t & 1;
+            const bool more = (t + 1) < nk;
+            if (more) load_slice(cur, (t + 1) * BK);
+            else if (has_next) load_slice(items[nx], 0);
+            const T* __restrict__ la = lds_a + cb * (BK * (BM + PAD));
+            const T* __restrict__ lb = lds_b + cb * (BK * (BN + PAD));
+#pragma unroll
+            for (int kk = 0; kk < BK; kk += 4) {
+                T af[FM], bf[FN];
+#pragma unroll
+                for (int i = 0; i < FM; ++i) af[i] = la[(kk + lk) * (BM + PAD) + wm0 + i * 16 + ar];
+#pragma unroll
+                for (int j = 0; j < FN; ++j) bf[j] = lb[(kk + lk) * (BN + PAD) + wn0 + j * 16 + lr];
+#pragma unroll
+                for (int i = 0; i < FM; ++i)
+#pragma unroll
+                    for (int j = 0; j < FN; ++j) acc[i][j] = M::mma(af[i], bf[j], acc[i][j]);
+            }
+            if (more) {
+                sa.store(lds_a + (cb ^ 1) * (BK * (BM + PAD)), tid);
+                sb.store(lds_b + (cb ^ 1) * (BK * (BN + PAD)), tid);
+            }
+            barrier();
+        }
+        // epilogue: lane holds rows 4*(lane/16)+r, column lane%16 of each block
+        const T alpha = p.alpha, beta = p.beta;
+#pragma unroll
+        for (int i = 0; i < FM; ++i) {
+#pragma unroll
+            for (int j = 0; j < FN; ++j) {
+                const int col = n0 + wn0 + j * 16 + lr;
+                const int row = m0 + wm0 + i * 16 + 4 * lk;
+                if (col < p.n) {
+                    T* cp = p.c + (int64_t)col * p.ldc + row;
+#pragma unroll
+                    for (int r = 0; r < 4; ++r)
+                        if (row + r < p.m) cp[r] = epilogue<T>(alpha, beta, acc[i][j][r], cp + r);
                 }
+            }
+        }
+        if (!has_next) return;
+        cur = items[nx];
+        it = nx;
+    }
+}
+
+// k == 0 or alpha == 0: C = beta * C (exact 0 for beta == 0) on 64 x 64 tiles
+template <typename T>
+__global__ __launch_bounds__(256) void grouped_scale_kernel(const TileItem<T>* __restrict__ items, int nitems) {
+    for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
+        const TileItem<T>& w = items[it];
+        for (int e = threadIdx.x; e < kScaleTile * kScaleTile; e += 256) {
+            const int i = w.m0 + e % kScaleTile, j = w.n0 + e / kScaleTile;
+            if (i < w.p.m && j < w.p.n) {
+                T* cp = w.p.c + (int64_t)j * w.p.ldc + i;
+                *cp = (w.p.beta == T(0)) ? T(0) : w.p.beta * *cp;
             }
         }
     }
 }
 
+// path id -> kernel: shape * 8 + vec * 4 + ta * 2 + tb (vec: 16-byte operand loads)
 template <typename T>
-__global__ __launch_bounds__(kGroupedThreads) __attribute__((amdgpu_waves_per_eu(2)))
-void grouped_gemm_kernel(const TileItem<T>* __restrict__ items, int nitems) {
-    __shared__ __attribute__((aligned(16))) T lds_a[tile_lds_a<T, GShape3>()];
-    __shared__ __attribute__((aligned(16))) T lds_b[tile_lds_b<T, GShape3>()];
-    int it = blockIdx.x;
-    if (it >= nitems) return;
-    // descriptors are uniform: scalar loads, the next one issued before this
-    // tile's operand loads so it has landed by the tile's first LDS wait
-    TileItem<T> cur = items[it];
-    for (;;) {
-        const int nx = it + gridDim.x;
-        TileItem<T> nxt;
-        if (nx < nitems) nxt = items[nx];
-        run_item<T>(cur, lds_a, lds_b);
-        if (nx >= nitems) return;
-        // every wave is done reading this tile's LDS; its C stores may still be in flight
-        asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
-        __builtin_amdgcn_s_barrier();
-        cur = nxt;
-        it = nx;
+const void* path_kernel(int path) {
+    constexpr int V = 16 / sizeof(T);
+    switch (path) {
+#define ZG_PATH4(SH, CFG, VB, VV)                                                                                   \
+    case SH * 8 + VB * 4 + 0: return reinterpret_cast<const void*>(grouped_path_kernel<T, CFG, false, false, VV>); \
+    case SH * 8 + VB * 4 + 1: return reinterpret_cast<const void*>(grouped_path_kernel<T, CFG, false, true, VV>);  \
+    case SH * 8 + VB * 4 + 2: return reinterpret_cast<const void*>(grouped_path_kernel<T, CFG, true, false, VV>);  \
+    case SH * 8 + VB * 4 + 3: return reinterpret_cast<const void*>(grouped_path_kernel<T, CFG, true, true, VV>);
+#define ZG_PATH(SH, CFG) ZG_PATH4(SH, CFG, 0, 1) ZG_PATH4(SH, CFG, 1, V)
+        ZG_PATH(0, GShape0) ZG_PATH(1, GShape1) ZG_PATH(2, GShape2) ZG_PATH(3, GShape3)
+#undef ZG_PATH
+#undef ZG_PATH4
+        default: return reinterpret_cast<const void*>(grouped_scale_kernel<T>);
     }
 }
+constexpr int kPathThreads[4] = {GShape0::NT, GShape1::NT, GShape2::NT, GShape3::NT};
+constexpr int kNumPaths = 32;  // plus the scale path
 
 // ---------------------------------------------------------------------------
 // tile choice per size bucket
@@ -133,7 +234,17 @@ struct zynum_gpu_grouped_plan {
     int nproblems = 0;
     int grid = 0;
     double flops = 0.0;
-    void* items = nullptr;  // device TileItem<T>[nitems], LPT order
+    void* items = nullptr;  // device TileItem<T>[nitems], grouped by path, LPT order within a path
+    struct Launch {
+        int path, first, count, grid;
+    };
+    std::vector<Launch> launches;  // one persistent kernel per non-empty path
+    // the launches run on up to kBranches streams forked from the caller's
+    // stream and joined back, so one kernel's tail overlaps the next one
+    static constexpr int kBranches = 4;
+    hipStream_t branch[kBranches] = {};
+    hipEvent_t fork = nullptr, join[kBranches] = {};
+    int nbranch = 0;
     hipStream_t capture_stream = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -158,18 +269,24 @@ bool problem_valid(const zynum_gpu_gemm_problem& p) {
 struct Seg {
     int start, size, tile;
 };
+// Smallest block a remainder is split into (a build knob: 32 measured faster
+// than 64 on the uneven 16..512 batch).
+#ifndef ZG_GROUPED_MIN_SPLIT
+#define ZG_GROUPED_MIN_SPLIT 32
+#endif
+constexpr int kMinSplit = ZG_GROUPED_MIN_SPLIT;
 static std::vector<Seg> split_dim(int dim, int t, bool mixed) {
     std::vector<Seg> out;
-    if (!mixed || t == 32) {
+    if (!mixed || t <= kMinSplit) {
         out.push_back({0, (dim + t - 1) / t * t, t});
         return out;
     }
     int pos = 0;
-    for (int b = t; b >= 32 && pos < dim; b /= 2) {
+    for (int b = t; b >= kMinSplit && pos < dim; b /= 2) {
         int cnt = (dim - pos) / b;
         const int rem = (dim - pos) - cnt * b;
         // round up at this size when the rest would not fit the smaller sizes with less waste
-        if (b == 32 || rem > b - 32) cnt += rem > 0;
+        if (b == kMinSplit || rem > b - kMinSplit) cnt += rem > 0;
         if (cnt > 0) {
             out.push_back({pos, cnt * b, b});
             pos += cnt * b;
@@ -190,6 +307,7 @@ template <typename T>
 int build_plan(const zynum_gpu_gemm_problem* probs, int count, int force_cfg, zynum_gpu_grouped_plan* plan) {
     std::vector<zg::TileItem<T>> items;
     std::vector<double> cost;
+    std::vector<int> prob;  // problem index of every tile
     for (int i = 0; i < count; ++i) {
         const zynum_gpu_gemm_problem& q = probs[i];
         if (!problem_valid(q)) return ZYNUM_GPU_DECLINE_ARGUMENT;
@@ -203,10 +321,15 @@ int build_plan(const zynum_gpu_gemm_problem* probs, int count, int force_cfg, zy
         g.a = static_cast<const T*>(q.a); g.b = static_cast<const T*>(q.b); g.c = static_cast<T*>(q.c);
         g.tiles_m = g.tiles_n = g.group_m = 1;
         const int ta = q.transa != ZYNUM_GPU_NO_TRANS, tb = q.transb != ZYNUM_GPU_NO_TRANS;
+        // 16-byte operand loads when both operands and leading dimensions allow them
+        constexpr int V = 16 / sizeof(T);
+        const int vec = ((reinterpret_cast<uintptr_t>(q.a) & 15) == 0 && (reinterpret_cast<uintptr_t>(q.b) & 15) == 0 &&
+                         q.lda % V == 0 && q.ldb % V == 0) ? 1 : 0;
         auto add = [&](int path, int m0, int n0, double c) {
             w.path = path; w.m0 = m0; w.n0 = n0;
             items.push_back(w);
             cost.push_back(c);
+            prob.push_back(i);
         };
         if (q.k == 0 || g.alpha == T(0)) {
             for (int n0 = 0; n0 < q.n; n0 += zg::kScaleTile)
@@ -231,15 +354,83 @@ int build_plan(const zynum_gpu_gemm_problem* probs, int count, int force_cfg, zy
                 const int th = zg::kShapeRows[sh], tw = zg::kShapeCols[sh];
                 for (int n0 = c.start; n0 < c.start + c.size; n0 += tw)
                     for (int m0 = r.start; m0 < r.start + r.size; m0 += th)
-                        add(sh * 4 + ta * 2 + tb, m0, n0, (double)th * tw * (double)q.k + 8192.0);
+                        add(sh * 8 + vec * 4 + ta * 2 + tb, m0, n0, (double)th * tw * (double)q.k + 8192.0);
             }
     }
-    // longest-processing-time order
+    // group by path (one persistent kernel each, largest tiles first), each
+    // group in longest-processing-time order
+    auto group = [](int path) { return path == zg::kScalePath ? -1 : path / 8; };
     std::vector<int> order(items.size());
     for (size_t i = 0; i < order.size(); ++i) order[i] = (int)i;
-    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return cost[x] > cost[y]; });
-    std::vector<zg::TileItem<T>> sorted(items.size());
-    for (size_t i = 0; i < order.size(); ++i) sorted[i] = items[order[i]];
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
+        const int gx = group(items[x].path), gy = group(items[y].path);
+        if (gx != gy) return gx > gy;
+        if (items[x].path != items[y].path) return items[x].path < items[y].path;
+        return cost[x] > cost[y];
+    });
+    std::vector<zg::TileItem<T>> grouped_items(items.size());
+    std::vector<double> grouped_cost(items.size());
+    std::vector<int> grouped_prob(items.size());
+    for (size_t i = 0; i < order.size(); ++i) {
+        grouped_items[i] = items[order[i]];
+        grouped_cost[i] = cost[order[i]];
+        grouped_prob[i] = prob[order[i]];
+    }
+    // XCD-aware placement within each path: workgroup w of a launch runs on
+    // XCD w % 8 and takes positions w, w + G, ... (G a multiple of 8), so
+    // position j runs on XCD j % 8.  Problems are assigned to XCDs greedily
+    // by cost (largest first) and each XCD's tiles fill its positions in LPT
+    // order, so all tiles of a problem share one L2; once an XCD's tiles run
+    // out its positions take the remaining tiles of the others.
+    std::vector<zg::TileItem<T>> sorted;
+    sorted.reserve(items.size());
+    plan->launches.clear();
+    for (size_t i = 0; i < grouped_items.size();) {
+        size_t j = i;
+        while (j < grouped_items.size() && grouped_items[j].path == grouped_items[i].path) ++j;
+        // problem cost within this path
+        std::vector<std::pair<double, int>> pc;  // (cost, problem)
+        {
+            std::vector<int> ids;
+            for (size_t x = i; x < j; ++x) ids.push_back(grouped_prob[x]);
+            std::sort(ids.begin(), ids.end());
+            ids.erase(std::unique(ids.begin(), ids.end()), ids.end());
+            std::vector<double> tot(ids.size(), 0.0);
+            for (size_t x = i; x < j; ++x)
+                tot[std::lower_bound(ids.begin(), ids.end(), grouped_prob[x]) - ids.begin()] += grouped_cost[x];
+            for (size_t q = 0; q < ids.size(); ++q) pc.push_back({tot[q], ids[q]});
+            std::stable_sort(pc.begin(), pc.end(), [](const auto& a, const auto& b) { return a.first > b.first; });
+        }
+        std::vector<double> load(zg::kXcds, 0.0);
+        std::vector<std::pair<int, int>> assign;  // (problem, xcd)
+        for (const auto& e : pc) {
+            const int x = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+            load[x] += e.first;
+            assign.push_back({e.second, x});
+        }
+        std::sort(assign.begin(), assign.end());
+        std::vector<std::vector<size_t>> queue(zg::kXcds);
+        for (size_t x = i; x < j; ++x) {  // LPT order is kept within every queue
+            const auto a = std::lower_bound(assign.begin(), assign.end(), std::make_pair(grouped_prob[x], -1));
+            queue[a->second].push_back(x);
+        }
+        std::vector<size_t> head(zg::kXcds, 0);
+        const size_t first = sorted.size();
+        for (size_t pos = 0; pos < j - i; ++pos) {
+            int q = (int)(pos % zg::kXcds);
+            if (head[q] >= queue[q].size()) {  // this XCD is done: take the longest remaining tile
+                int best = -1;
+                for (int r = 0; r < zg::kXcds; ++r)
+                    if (head[r] < queue[r].size() &&
+                        (best < 0 || grouped_cost[queue[r][head[r]]] > grouped_cost[queue[best][head[best]]]))
+                        best = r;
+                q = best;
+            }
+            sorted.push_back(grouped_items[queue[q][head[q]++]]);
+        }
+        plan->launches.push_back({grouped_items[i].path, (int)first, (int)(j - i), 1});
+        i = j;
+    }
 
     plan->nitems = (int)sorted.size();
     plan->nproblems = count;
@@ -257,23 +448,61 @@ int build_plan(const zynum_gpu_gemm_problem* probs, int count, int force_cfg, zy
         if (hipGetDeviceProperties(&prop, dev) == hipSuccess) cus = prop.multiProcessorCount;
     }
     (void)hipGetLastError();
-    // as many persistent workgroups as are resident at once (register and LDS
-    // limits of this kernel), so the round-robin share of every one is running
-    int per_cu = 1;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(zg::grouped_gemm_kernel<T>),
-                                                     zg::kGroupedThreads, 0) != hipSuccess || per_cu < 1)
-        per_cu = 1;
-    (void)hipGetLastError();
-    plan->grid = std::max(1, std::min(plan->nitems, per_cu * cus));
+    // per path: as many persistent workgroups as are resident at once
+    // (register and LDS limits of that kernel), so every round-robin share runs
+    plan->grid = 0;
+    for (auto& l : plan->launches) {
+        const int threads = l.path == zg::kScalePath ? 256 : zg::kPathThreads[l.path / 8];
+        int per_cu = 1;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, zg::path_kernel<T>(l.path), threads, 0) != hipSuccess ||
+            per_cu < 1)
+            per_cu = 1;
+        (void)hipGetLastError();
+        l.grid = std::max(1, std::min(l.count, per_cu * cus));
+        plan->grid = std::max(plan->grid, l.grid);
+    }
     return ZYNUM_GPU_OK;
 }
 
 template <typename T>
-hipError_t launch_plan(const zynum_gpu_grouped_plan* plan, hipStream_t s) {
-    if (plan->nitems == 0) return hipSuccess;
-    zg::grouped_gemm_kernel<T><<<plan->grid, zg::kGroupedThreads, 0, s>>>(static_cast<const zg::TileItem<T>*>(plan->items),
-                                                          plan->nitems);
-    return hipGetLastError();
+hipError_t launch_plan(const zynum_gpu_grouped_plan* plan, hipStream_t caller) {
+    constexpr int V = 16 / sizeof(T);
+    const zg::TileItem<T>* base = static_cast<const zg::TileItem<T>*>(plan->items);
+    const int nb = plan->nbranch;
+    if (nb > 1) {
+        hipError_t e = hipEventRecord(plan->fork, caller);
+        for (int b = 0; b < nb && e == hipSuccess; ++b) e = hipStreamWaitEvent(plan->branch[b], plan->fork, 0);
+        if (e != hipSuccess) return e;
+    }
+    for (size_t li = 0; li < plan->launches.size(); ++li) {
+        const auto& l = plan->launches[li];
+        hipStream_t s = nb > 1 ? plan->branch[li % nb] : caller;
+        const zg::TileItem<T>* it = base + l.first;
+        switch (l.path) {
+#define ZG_LAUNCH4(SH, CFG, VB, VV)                                                                       \
+    case SH * 8 + VB * 4 + 0:                                                                             \
+        zg::grouped_path_kernel<T, CFG, false, false, VV><<<l.grid, CFG::NT, 0, s>>>(it, l.count); break; \
+    case SH * 8 + VB * 4 + 1:                                                                             \
+        zg::grouped_path_kernel<T, CFG, false, true, VV><<<l.grid, CFG::NT, 0, s>>>(it, l.count); break;  \
+    case SH * 8 + VB * 4 + 2:                                                                             \
+        zg::grouped_path_kernel<T, CFG, true, false, VV><<<l.grid, CFG::NT, 0, s>>>(it, l.count); break;  \
+    case SH * 8 + VB * 4 + 3:                                                                             \
+        zg::grouped_path_kernel<T, CFG, true, true, VV><<<l.grid, CFG::NT, 0, s>>>(it, l.count); break;
+#define ZG_LAUNCH(SH, CFG) ZG_LAUNCH4(SH, CFG, 0, 1) ZG_LAUNCH4(SH, CFG, 1, V)
+            ZG_LAUNCH(0, zg::GShape0) ZG_LAUNCH(1, zg::GShape1) ZG_LAUNCH(2, zg::GShape2) ZG_LAUNCH(3, zg::GShape3)
+#undef ZG_LAUNCH
+#undef ZG_LAUNCH4
+            default: zg::grouped_scale_kernel<T><<<l.grid, 256, 0, s>>>(it, l.count);
+        }
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    for (int b = 0; b < nb && nb > 1; ++b) {
+        hipError_t e = hipEventRecord(plan->join[b], plan->branch[b]);
+        if (e == hipSuccess) e = hipStreamWaitEvent(caller, plan->join[b], 0);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
 }
 
 hipError_t launch_any(const zynum_gpu_grouped_plan* plan, hipStream_t s) {
@@ -294,8 +523,22 @@ int zynum_gpu_grouped_plan_create(int scalar, const zynum_gpu_gemm_problem* prob
     if (!zg::device_ready()) return ZYNUM_GPU_DECLINE_NO_DEVICE;
     zynum_gpu_grouped_plan* plan = new zynum_gpu_grouped_plan;
     plan->scalar = scalar;
-    const int st = scalar == ZYNUM_GPU_F64 ? build_plan<double>(problems, count, tile - 1, plan)
-                                           : build_plan<float>(problems, count, tile - 1, plan);
+    int st = scalar == ZYNUM_GPU_F64 ? build_plan<double>(problems, count, tile - 1, plan)
+                                     : build_plan<float>(problems, count, tile - 1, plan);
+    if (st == ZYNUM_GPU_OK) {
+        plan->nbranch = std::min<int>(zynum_gpu_grouped_plan::kBranches, (int)plan->launches.size());
+        if (plan->nbranch > 1) {
+            hipError_t e = hipEventCreateWithFlags(&plan->fork, hipEventDisableTiming);
+            for (int b = 0; b < plan->nbranch && e == hipSuccess; ++b) {
+                e = hipStreamCreateWithFlags(&plan->branch[b], hipStreamNonBlocking);
+                if (e == hipSuccess) e = hipEventCreateWithFlags(&plan->join[b], hipEventDisableTiming);
+            }
+            if (e != hipSuccess) {
+                (void)hipGetLastError();
+                st = ZYNUM_GPU_DECLINE_ALLOC;
+            }
+        }
+    }
     if (st != ZYNUM_GPU_OK) {
         zynum_gpu_grouped_plan_destroy(plan);
         return st;
@@ -340,6 +583,11 @@ void zynum_gpu_grouped_plan_destroy(zynum_gpu_grouped_plan* plan) {
     if (plan->exec) (void)hipGraphExecDestroy(plan->exec);
     if (plan->graph) (void)hipGraphDestroy(plan->graph);
     if (plan->capture_stream) (void)hipStreamDestroy(plan->capture_stream);
+    for (int b = 0; b < zynum_gpu_grouped_plan::kBranches; ++b) {
+        if (plan->branch[b]) (void)hipStreamDestroy(plan->branch[b]);
+        if (plan->join[b]) (void)hipEventDestroy(plan->join[b]);
+    }
+    if (plan->fork) (void)hipEventDestroy(plan->fork);
     if (plan->items) {
         (void)hipDeviceSynchronize();
         (void)hipFree(plan->items);
