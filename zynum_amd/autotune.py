@@ -30,7 +30,7 @@ import numpy as np
 from . import _lib, blas, grouped, synth
 
 CSRC = Path(__file__).resolve().parent / "csrc"
-PLAIN_TILES = 8          # kTileConfigs
+PLAIN_TILES = 11         # kTileConfigs
 GROUPED_TILES = 3        # kGroupedCfgs
 I8_TILES = 14            # kI8Tiles
 PLAIN_SIZES = (8192, 2048, 1024)

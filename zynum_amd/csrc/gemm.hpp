@@ -29,7 +29,10 @@ enum TileConfig : int {
     kTile128x128k32 = 5,  // 2x2 waves of 64x64, BK 32
     kTile128x128w8 = 6,   // 512 threads, 4x2 waves of 32x64
     kTile64x64k32 = 7,    // as 0 with BK 32
-    kTileConfigs = 8
+    kTile256x128w8 = 8,   // 512 threads, 4x2 waves of 64x64, one workgroup per CU
+    kTile128x256w8 = 9,   // 512 threads, 2x4 waves of 64x64, one workgroup per CU
+    kTile128x128w8k8 = 10,  // as 6 with BK 8 (half the LDS: more workgroups per CU)
+    kTileConfigs = 11
 };
 
 // Tile choice for one GEMM shape from the measured tuning table

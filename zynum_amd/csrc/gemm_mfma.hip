@@ -108,6 +108,9 @@ hipError_t gemm_launch(const GemmDesc<T>& d, hipStream_t stream, int tile) {
         case kTile128x128k32: return launch_shape<T, 128, 128, 32, 2, 2, 1>(d, stream);
         case kTile128x128w8: return launch_shape<T, 128, 128, 16, 4, 2, 2>(d, stream);
         case kTile64x64k32: return launch_shape<T, 64, 64, 32, 2, 2, 0>(d, stream);
+        case kTile256x128w8: return launch_shape<T, 256, 128, 16, 4, 2, 1>(d, stream);
+        case kTile128x256w8: return launch_shape<T, 128, 256, 16, 2, 4, 1>(d, stream);
+        case kTile128x128w8k8: return launch_shape<T, 128, 128, 8, 4, 2, 2>(d, stream);
         default: return hipErrorInvalidValue;
     }
 }
