@@ -42,6 +42,8 @@ def parse_args():
     p.add_argument("--no-variants", action="store_true", help="skip the transpose/beta/SGEMM variant table")
     p.add_argument("--no-grouped", action="store_true", help="skip the grouped uneven-GEMM leg")
     p.add_argument("--no-level3", action="store_true", help="skip the DTRSM/DSYRK leg")
+    p.add_argument("--no-complex", action="store_true", help="skip the ZGEMM/CGEMM leg")
+    p.add_argument("--only-complex", action="store_true", help="headline plus the ZGEMM/CGEMM leg (profiler runs)")
     p.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU-baseline sample time")
     p.add_argument("--only-headline", action="store_true", help="plain FP64 leg only (for profiler runs)")
     p.add_argument("--calibrate-copy", action="store_true",
@@ -63,6 +65,8 @@ def read_traffic(name: str):
 def main() -> int:
     args = parse_args()
     if args.only_headline:
+        args.no_cpu = args.no_split = args.no_variants = args.no_grouped = args.no_level3 = args.no_complex = True
+    if args.only_complex:
         args.no_cpu = args.no_split = args.no_variants = args.no_grouped = args.no_level3 = True
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -167,60 +171,71 @@ def main() -> int:
         },
     }
 
-    # ---------------- variants (transposes, beta, SGEMM) -----------------
-    if not args.no_variants:
-        variants = {}
-        C1 = torch.randn(ld * n, dtype=torch.float64, device=dev)
-        for ta, tb in ((False, False), (False, True), (True, False), (True, True)):
-            for beta in (0.0, 1.0):
-                name = f"{'T' if ta else 'N'}{'T' if tb else 'N'}_beta{int(beta)}"
-                step(ta=ta, tb=tb, beta=beta, c=C1)
-                s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-                s.record(stream)
-                for _ in range(3):
+    # torch work (allocations, copies, reductions) and the library calls share
+    # one stream, so every check reads finished results
+    with torch.cuda.stream(stream):
+        # ---------------- variants (transposes, beta, SGEMM) -----------------
+        if not args.no_variants:
+            variants = {}
+            C1 = torch.randn(ld * n, dtype=torch.float64, device=dev)
+            for ta, tb in ((False, False), (False, True), (True, False), (True, True)):
+                for beta in (0.0, 1.0):
+                    name = f"{'T' if ta else 'N'}{'T' if tb else 'N'}_beta{int(beta)}"
                     step(ta=ta, tb=tb, beta=beta, c=C1)
-                e.record(stream)
-                torch.cuda.synchronize()
-                variants[f"dgemm_{name}"] = round(3 * flop / (s.elapsed_time(e) * 1e-3) / 1e12, 2)
-        del C1
-        A32, B32 = A.float(), B.float()
-        C32 = torch.zeros(ld * n, dtype=torch.float32, device=dev)
-        step(scalar=_lib.F32, a=A32, b=B32, c=C32)
-        s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        s.record(stream)
-        for _ in range(5):
+                    s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                    s.record(stream)
+                    for _ in range(3):
+                        step(ta=ta, tb=tb, beta=beta, c=C1)
+                    e.record(stream)
+                    torch.cuda.synchronize()
+                    variants[f"dgemm_{name}"] = round(3 * flop / (s.elapsed_time(e) * 1e-3) / 1e12, 2)
+            del C1
+            A32, B32 = A.float(), B.float()
+            C32 = torch.zeros(ld * n, dtype=torch.float32, device=dev)
             step(scalar=_lib.F32, a=A32, b=B32, c=C32)
-        e.record(stream)
-        torch.cuda.synchronize()
-        variants["sgemm_NN_beta0"] = round(5 * flop / (s.elapsed_time(e) * 1e-3) / 1e12, 2)
-        variants["sgemm_frac_of_157.3TF"] = round(variants["sgemm_NN_beta0"] / 157.3, 4)
-        del A32, B32, C32
-        result["variants_tflops"] = variants
-        log(f"variants: {variants}")
+            s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            s.record(stream)
+            for _ in range(5):
+                step(scalar=_lib.F32, a=A32, b=B32, c=C32)
+            e.record(stream)
+            torch.cuda.synchronize()
+            variants["sgemm_NN_beta0"] = round(5 * flop / (s.elapsed_time(e) * 1e-3) / 1e12, 2)
+            variants["sgemm_frac_of_157.3TF"] = round(variants["sgemm_NN_beta0"] / 157.3, 4)
+            del A32, B32, C32
+            result["variants_tflops"] = variants
+            log(f"variants: {variants}")
 
-    # ---------------- split precision (INT8 / CRT) -----------------
-    if not args.no_split:
-        try:
-            result["split_int8"] = split_leg(torch, lib, A, B, n, ld, stream, achieved)
-        except Exception as exc:  # reported, never hidden
-            result["split_int8"] = {"error": str(exc)}
-            log(f"split leg failed: {exc}")
+        # ---------------- split precision (INT8 / CRT) -----------------
+        if not args.no_split:
+            try:
+                result["split_int8"] = split_leg(torch, lib, A, B, n, ld, stream, achieved)
+            except Exception as exc:  # reported, never hidden
+                result["split_int8"] = {"error": str(exc)}
+                log(f"split leg failed: {exc}")
 
-    # ---------------- grouped uneven GEMM -----------------
-    if not args.no_grouped:
-        try:
-            result["grouped"] = grouped_leg(torch, stream, achieved)
-        except Exception as exc:
-            result["grouped"] = {"error": str(exc)}
-            log(f"grouped leg failed: {exc}")
+        # ---------------- grouped uneven GEMM -----------------
+        if not args.no_grouped:
+            try:
+                result["grouped"] = grouped_leg(torch, stream, achieved)
+            except Exception as exc:
+                result["grouped"] = {"error": str(exc)}
+                log(f"grouped leg failed: {exc}")
 
-    # ---------------- DTRSM / DSYRK -----------------
-    if not args.no_level3:
-        try:
-            result["level3"] = level3_leg(torch, lib, n, stream)
-        except Exception as exc:
-            result["level3"] = {"error": str(exc)}
-            log(f"level3 leg failed: {exc}")
+        # ---------------- DTRSM / DSYRK -----------------
+        if not args.no_level3:
+            try:
+                result["level3"] = level3_leg(torch, lib, n, stream)
+            except Exception as exc:
+                result["level3"] = {"error": str(exc)}
+                log(f"level3 leg failed: {exc}")
+
+        # ---------------- complex GEMM -----------------
+        if not args.no_complex:
+            try:
+                result["complex"] = complex_leg(torch, lib, n, stream, achieved)
+            except Exception as exc:
+                result["complex"] = {"error": str(exc)}
+                log(f"complex leg failed: {exc}")
 
     # ---------------- CPU baseline -----------------
     if not args.no_cpu:
@@ -418,6 +433,102 @@ def level3_leg(torch, lib, n, stream):
         del C
     out["dsyrk"] = res
     log(f"level3: {out}")
+    return out
+
+
+def complex_leg(torch, lib, n, stream, plain_tf):
+    """ZGEMM / CGEMM M=N=K=n (uniform [-1,1) real and imaginary parts) through
+    the request ABI: 8MNK real flops per call; errors on a 1024 x 1024 corner
+    against the device double-double product of the real and imaginary parts."""
+    from zynum_amd import _lib, blas
+
+    dev = torch.device("cuda:0")
+    sh = stream.cuda_stream
+    ld = n + 8
+    g = torch.Generator(device=dev)
+    g.manual_seed(21)
+
+    def cplx(dtype):
+        re = torch.rand((n, ld), dtype=torch.float64, device=dev, generator=g) * 2 - 1
+        im = torch.rand((n, ld), dtype=torch.float64, device=dev, generator=g) * 2 - 1
+        return torch.complex(re, im).to(dtype)  # row j of the tensor = column j (column-major)
+
+    A, B = cplx(torch.complex128), cplx(torch.complex128)
+    flop = 8.0 * n ** 3
+    out = {"workload": f"ZGEMM/CGEMM M=N=K={n}, ld={ld}, alpha=1, beta=0, one real MFMA GEMM on the 2x2-block "
+                       "expansion of op(A) per call; inputs resident in HBM"}
+
+    def timed(scalar, ta, tb, a, b, c, reps, alpha=1.0, beta=0.0):
+        def go():
+            st = blas.gemm_request(ta, tb, n, n, n, alpha, a.data_ptr(), ld, b.data_ptr(), ld, beta, c.data_ptr(),
+                                   ld, scalar, stream=sh, flags=_lib.FLAG_ASYNC)
+            _lib.check(st, "complex gemm")
+        go()
+        torch.cuda.synchronize()
+        s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        s.record(stream)
+        for _ in range(reps):
+            go()
+        e.record(stream)
+        torch.cuda.synchronize()
+        return s.elapsed_time(e) / reps
+
+    def corner_error(C, ta, tb, S=1024):
+        """Max error of C[:S, :S] against dd(op(A) op(B)) built from real GEMMs."""
+        Ar = A[:, :S] if ta == "N" else A[:S, :n].T     # (k, S): [p, i] = op(A)(i, p)
+        Ai = Ar.imag * (-1.0 if ta == "C" else 1.0)
+        Ar = Ar.real
+        Ar, Ai = Ar.contiguous(), Ai.contiguous()        # (k, S): column-major S x k, ld S
+        Bs = B[:S, :n] if tb == "N" else B[:, :S].T      # (S, k): [j, p] = op(B)(p, j), column-major k x S
+        Bi = Bs.imag * (-1.0 if tb == "C" else 1.0)
+        Br, Bi = Bs.real.contiguous(), Bi.contiguous()
+        parts = {}
+        for name, x, y in (("rr", Ar, Br), ("ii", Ai, Bi), ("ri", Ar, Bi), ("ir", Ai, Br)):
+            hi = torch.zeros(S * S, dtype=torch.float64, device=dev)
+            lo = torch.zeros(S * S, dtype=torch.float64, device=dev)
+            st = lib.zynum_gpu_dd_gemm(b"N", b"N", S, S, n, 1.0, x.data_ptr(), S, y.data_ptr(), n, 0.0, None, S,
+                                       hi.data_ptr(), lo.data_ptr(), S, sh)
+            _lib.check(st, "dd reference")
+            parts[name] = (hi.view(S, S), lo.view(S, S))
+        ref_re = (parts["rr"][0] - parts["ii"][0]) + (parts["rr"][1] - parts["ii"][1])
+        ref_im = (parts["ri"][0] + parts["ir"][0]) + (parts["ri"][1] + parts["ir"][1])
+        got = C[:S, :S].to(torch.complex128)
+        err = torch.maximum((got.real - ref_re).abs(), (got.imag - ref_im).abs())
+        mag = torch.maximum(ref_re.abs(), ref_im.abs())
+        return float(err.max()), float(err.max() / mag.max())
+
+    for tag, ta, tb in (("zgemm_NN", "N", "N"), ("zgemm_CN", "C", "N"), ("zgemm_NT", "N", "T")):
+        C = torch.zeros((n, ld), dtype=torch.complex128, device=dev)
+        ms = timed(_lib.C64, ta, tb, A, B, C, 3)
+        err, nerr = corner_error(C, ta, tb)
+        out[tag] = {"ms": round(ms, 3), "tflops": round(flop / (ms * 1e-3) / 1e12, 2),
+                    "frac_of_78.6TF": round(flop / (ms * 1e-3) / 1e12 / 78.6, 4),
+                    "max_abs_err_vs_dd_1024_corner": err, "normwise_err_vs_dd_1024_corner": nerr}
+        del C
+    out["zgemm_NN"]["vs_plain_dgemm"] = round(out["zgemm_NN"]["tflops"] / plain_tf, 3)
+    C = torch.zeros((n, ld), dtype=torch.complex128, device=dev)
+    ms = timed(_lib.C64, "N", "N", A, B, C, 3, alpha=0.6 - 0.4j, beta=-0.2 + 0.3j)
+    out["zgemm_NN_complex_alpha_beta"] = {"ms": round(ms, 3), "tflops": round(flop / (ms * 1e-3) / 1e12, 2)}
+    # the library's own complex GEMM on the same operands, for reference (torch.matmul -> rocBLAS/hipBLASLt)
+    At, Bt = A[:, :n].contiguous(), B[:, :n].contiguous()
+    torch.matmul(Bt, At)
+    torch.cuda.synchronize()
+    s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    s.record()
+    for _ in range(2):
+        torch.matmul(Bt, At)
+    e.record()
+    torch.cuda.synchronize()
+    out["torch_matmul_complex128_tflops"] = round(flop / (s.elapsed_time(e) / 2 * 1e-3) / 1e12, 2)
+    del C, At, Bt
+    A32, B32 = A.to(torch.complex64), B.to(torch.complex64)
+    del A, B
+    C32 = torch.zeros((n, ld), dtype=torch.complex64, device=dev)
+    ms = timed(_lib.C32, "N", "N", A32, B32, C32, 3)
+    out["cgemm_NN"] = {"ms": round(ms, 3), "tflops": round(flop / (ms * 1e-3) / 1e12, 2),
+                       "frac_of_157.3TF": round(flop / (ms * 1e-3) / 1e12 / 157.3, 4)}
+    del A32, B32, C32
+    log(f"complex: {out}")
     return out
 
 

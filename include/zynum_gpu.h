@@ -14,6 +14,11 @@
  *                              src/blas/abi/cblas.zig:1415-1439
  *   sgemm_ / dgemm_            include/zynum/blas/blas.h:161-162,
  *                              src/blas/abi/fortran.zig:760-767
+ *   cgemm_ / zgemm_            include/zynum/blas/blas.h:163-164,
+ *                              src/blas/abi/fortran.zig:768-775
+ *   cblas_cgemm / cblas_zgemm  include/zynum/blas/cblas.h:1370-1402,
+ *                              src/blas/abi/cblas.zig:1441-1465
+ *   zynum_blas_complex_*       include/zynum/blas/blas.h:15-28
  *   dsyrk_ / cblas_dsyrk       src/blas/abi/fortran.zig:806-809,
  *                              src/blas/abi/cblas.zig:1550-1559, cblas.h:1514
  *   dtrsm_ / cblas_dtrsm       src/blas/abi/fortran.zig:873-876,
@@ -38,6 +43,19 @@ extern "C" {
 #pragma GCC visibility push(default)
 
 typedef int32_t zynum_blas_int;
+#ifndef ZYNUM_BLAS_TYPES_DEFINED
+#define ZYNUM_BLAS_TYPES_DEFINED
+typedef struct {
+    float real;
+    float imag;
+} zynum_blas_complex_float;
+typedef struct {
+    double real;
+    double imag;
+} zynum_blas_complex_double;
+typedef zynum_blas_complex_float zynum_blas_complexF32;
+typedef zynum_blas_complex_double zynum_blas_complexF64;
+#endif
 
 /* ---------------------------------------------------------------------- */
 /* Part 1: Zynum public BLAS symbols                                       */
@@ -55,6 +73,11 @@ void cblas_sgemm(int layout, int transa, int transb, int m, int n, int k, float 
                  const float *a, int lda, const float *b, int ldb, float beta, float *c, int ldc);
 void cblas_dgemm(int layout, int transa, int transb, int m, int n, int k, double alpha,
                  const double *a, int lda, const double *b, int ldb, double beta, double *c, int ldc);
+/* complex forms: alpha and beta point at one complex scalar; a, b, c at interleaved (re, im) storage */
+void cblas_cgemm(int layout, int transa, int transb, int m, int n, int k, const void *alpha,
+                 const void *a, int lda, const void *b, int ldb, const void *beta, void *c, int ldc);
+void cblas_zgemm(int layout, int transa, int transb, int m, int n, int k, const void *alpha,
+                 const void *a, int lda, const void *b, int ldb, const void *beta, void *c, int ldc);
 void cblas_dsyrk(int layout, int uplo, int trans, int n, int k, double alpha, const double *a,
                  int lda, double beta, double *c, int ldc);
 void cblas_dtrsm(int layout, int side, int uplo, int transa, int diag, int m, int n, double alpha,
@@ -67,6 +90,18 @@ void sgemm_(const char *transa, const char *transb, const zynum_blas_int *m, con
 void dgemm_(const char *transa, const char *transb, const zynum_blas_int *m, const zynum_blas_int *n,
             const zynum_blas_int *k, const double *alpha, const double *a, const zynum_blas_int *lda,
             const double *b, const zynum_blas_int *ldb, const double *beta, double *c,
+            const zynum_blas_int *ldc);
+void cgemm_(const char *transa, const char *transb, const zynum_blas_int *m, const zynum_blas_int *n,
+            const zynum_blas_int *k, const zynum_blas_complex_float *alpha,
+            const zynum_blas_complex_float *a, const zynum_blas_int *lda,
+            const zynum_blas_complex_float *b, const zynum_blas_int *ldb,
+            const zynum_blas_complex_float *beta, zynum_blas_complex_float *c,
+            const zynum_blas_int *ldc);
+void zgemm_(const char *transa, const char *transb, const zynum_blas_int *m, const zynum_blas_int *n,
+            const zynum_blas_int *k, const zynum_blas_complex_double *alpha,
+            const zynum_blas_complex_double *a, const zynum_blas_int *lda,
+            const zynum_blas_complex_double *b, const zynum_blas_int *ldb,
+            const zynum_blas_complex_double *beta, zynum_blas_complex_double *c,
             const zynum_blas_int *ldc);
 void ssyrk_(const char *uplo, const char *trans, const zynum_blas_int *n, const zynum_blas_int *k,
             const float *alpha, const float *a, const zynum_blas_int *lda, const float *beta,
@@ -103,7 +138,8 @@ enum {
     ZYNUM_GPU_ERR_LAUNCH = 6           /* HIP error after launch; outputs undefined */
 };
 
-enum { ZYNUM_GPU_F32 = 0, ZYNUM_GPU_F64 = 1 };
+/* scalar codes of the reference's structured ABI (x86_64_structured_abi.zig:14-19) */
+enum { ZYNUM_GPU_F32 = 0, ZYNUM_GPU_F64 = 1, ZYNUM_GPU_C32 = 2, ZYNUM_GPU_C64 = 3 };
 enum { ZYNUM_GPU_NO_TRANS = 0, ZYNUM_GPU_TRANS = 1, ZYNUM_GPU_CONJ_TRANS = 2 };
 
 /* Precision modes of a DGEMM request. */
@@ -127,13 +163,13 @@ enum {
 typedef struct zynum_gpu_gemm_request {
     uint8_t transa;      /* ZYNUM_GPU_NO_TRANS / TRANS / CONJ_TRANS */
     uint8_t transb;
-    uint8_t scalar;      /* ZYNUM_GPU_F32 / ZYNUM_GPU_F64 */
+    uint8_t scalar;      /* ZYNUM_GPU_F32 / F64 / C32 / C64 */
     uint8_t reserved0[5];
     int32_t m, n, k, lda, ldb, ldc;
     uint64_t alpha_re;   /* bit pattern of alpha (double, or float in the low 32 bits) */
-    uint64_t alpha_im;   /* must be 0 */
+    uint64_t alpha_im;   /* imaginary parts: complex scalars only, must be 0 for F32 / F64 */
     uint64_t beta_re;
-    uint64_t beta_im;    /* must be 0 */
+    uint64_t beta_im;
     const void *a;
     const void *b;
     void *c;
@@ -148,7 +184,10 @@ typedef struct zynum_gpu_gemm_request {
 } zynum_gpu_gemm_request; /* 104 bytes */
 
 /* Run one GEMM request: C = alpha*op(A)*op(B) + beta*C (column-major).
-   Arguments follow dgemm_ rules (gemmError, fortran.zig:88-103). */
+   Arguments follow dgemm_ rules (gemmError, fortran.zig:88-103).  Complex
+   requests (C32 / C64, interleaved storage, leading dimensions in complex
+   elements) take CONJ_TRANS as the conjugate transpose and mode PLAIN only; they
+   run one real MFMA GEMM on op(A) expanded into 2x2 real blocks. */
 int zynum_gpu_gemm(const zynum_gpu_gemm_request *request);
 
 /* One problem of a grouped GEMM batch (device pointers, column-major). */

@@ -101,3 +101,57 @@ def force_isa(v: int) -> int:
 
 def now() -> float:
     return load().oracle_now()
+
+
+def _cfuncs():
+    lib = load()
+    if not getattr(lib, "_complex_bound", False):
+        vp, i, ch = C.c_void_p, C.c_int, C.c_char
+        for name in ("oracle_zgemm", "oracle_cgemm"):
+            getattr(lib, name).argtypes = [ch, ch, i, i, i, vp, vp, i, vp, i, vp, vp, i]
+        for name in ("oracle_zgemm_entries", "oracle_cgemm_entries", "oracle_zdd_gemm_entries"):
+            extra = [vp] if name == "oracle_zdd_gemm_entries" else []
+            getattr(lib, name).argtypes = [ch, ch, i, vp, vp, i, vp, i, vp, vp, i, vp, vp, C.c_int64, vp] + extra
+        lib._complex_bound = True
+    return lib
+
+
+def _scal(v, dtype) -> np.ndarray:
+    v = complex(v)
+    return np.array([v.real, v.imag], dtype=dtype)
+
+
+def zgemm(ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc) -> None:
+    """In-place on ``c``; a, b, c are complex128 flat column-major buffers,
+    leading dimensions in complex elements (gemm.zig:1234-1246)."""
+    al, be = _scal(alpha, np.float64), _scal(beta, np.float64)
+    _cfuncs().oracle_zgemm(_c(ta), _c(tb), m, n, k, _p(al), _p(a), lda, _p(b), ldb, _p(be), _p(c), ldc)
+
+
+def cgemm(ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc) -> None:
+    """complex64 form of :func:`zgemm`."""
+    al, be = _scal(alpha, np.float32), _scal(beta, np.float32)
+    _cfuncs().oracle_cgemm(_c(ta), _c(tb), m, n, k, _p(al), _p(a), lda, _p(b), ldb, _p(be), _p(c), ldc)
+
+
+def zgemm_entries(ta, tb, k, alpha, a, lda, b, ldb, beta, c0, ldc, rows, cols) -> np.ndarray:
+    """Oracle value of selected entries of a complex128 product (c0 may be None when beta == 0)."""
+    rows = np.ascontiguousarray(rows, dtype=np.int64)
+    cols = np.ascontiguousarray(cols, dtype=np.int64)
+    out = np.empty(rows.size, dtype=np.complex128)
+    al, be = _scal(alpha, np.float64), _scal(beta, np.float64)
+    _cfuncs().oracle_zgemm_entries(_c(ta), _c(tb), k, _p(al), _p(a), lda, _p(b), ldb, _p(be), _p(c0), ldc,
+                                   _p(rows), _p(cols), rows.size, _p(out))
+    return out
+
+
+def zdd_gemm_entries(ta, tb, k, alpha, a, lda, b, ldb, beta, c0, ldc, rows, cols):
+    """Double-double value (hi, lo as complex128 arrays) of selected entries of a complex product."""
+    rows = np.ascontiguousarray(rows, dtype=np.int64)
+    cols = np.ascontiguousarray(cols, dtype=np.int64)
+    hi = np.empty(rows.size, dtype=np.complex128)
+    lo = np.empty(rows.size, dtype=np.complex128)
+    al, be = _scal(alpha, np.float64), _scal(beta, np.float64)
+    _cfuncs().oracle_zdd_gemm_entries(_c(ta), _c(tb), k, _p(al), _p(a), lda, _p(b), ldb, _p(be), _p(c0), ldc,
+                                      _p(rows), _p(cols), rows.size, _p(hi), _p(lo))
+    return hi, lo

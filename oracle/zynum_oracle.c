@@ -21,6 +21,8 @@
  *   - TRSM: core/matrix_matrix/triangular.zig:52-63,72-84,103-118,233-249 and
  *           core/matrix_vector/triangular.zig:44-57,362-412
  *   - SYRK: core/matrix_matrix/symmetric.zig:180-200,237-279
+ *   - complex GEMM (CGEMM / ZGEMM): gemm.zig:1234-1246,1404-1418 with the
+ *           complex scalar rules of core/shared/scalar.zig (see below)
  *
  * Pinning (see tests/test_oracle.py): known-answer vectors from the
  * reference's own tests, bitwise equality against exact integer arithmetic on
@@ -582,6 +584,114 @@ void oracle_dd_gemm_entries(char ta, char tb, int k, double alpha, const double 
         if (beta != 0.0) s = dd_add(s, two_prod(beta, c0[mi(ldc, i, j)]));
         out_hi[e] = s.hi;
         out_lo[e] = s.lo;
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Complex GEMM.  Storage is interleaved (re, im); leading dimensions count    */
+/* complex elements.                                                          */
+/*   - scalar add / mul / conj: core/shared/scalar.zig:56-60,77-82,91-95       */
+/*     (four separately rounded products, no fused steps)                     */
+/*   - element access with conjugation: core/matrix_vector/access.zig:23-29    */
+/*   - k == 0 or alpha == 0 (both parts): gemm.zig:1407-1414                  */
+/*   - portable loop: gemm.zig:1234-1246, sum over p ascending, then           */
+/*     alpha*sum + (beta == 0 ? 0 : beta*c)                                   */
+/* The reference's tuned complex routes (expanded real, 3M, compact,          */
+/* gemm.zig:1260-1280) regroup the same sum; their exact roundings are        */
+/* "parity unpinned" here and the tests compare within a stated bound.        */
+/* ------------------------------------------------------------------------ */
+#define DEFINE_CGEMM(NAME, ENTRIES, T)                                                         \
+    static inline void NAME##_mul(T ar, T ai, T br, T bi, T *rr, T *ri) {                    \
+        *rr = ar * br - ai * bi;                                                               \
+        *ri = ar * bi + ai * br;                                                               \
+    }                                                                                          \
+    static inline void NAME##_at(char t, const T *x, int ld, size_t r, size_t c, T *re, T *im) { \
+        const T *p = is_no_trans(t) ? &x[2 * mi(ld, r, c)] : &x[2 * mi(ld, c, r)];             \
+        *re = p[0];                                                                            \
+        *im = (t == 'C' || t == 'c') ? -p[1] : p[1];                                           \
+    }                                                                                          \
+    static inline void NAME##_one(char ta, char tb, int k, const T *alpha, const T *a, int lda, \
+                                  const T *b, int ldb, const T *beta, const T *cold, size_t i,  \
+                                  size_t j, T *out) {                                          \
+        int bzero = beta[0] == (T)0 && beta[1] == (T)0;                                        \
+        T br_ = 0, bi_ = 0;                                                                    \
+        if (!bzero) NAME##_mul(beta[0], beta[1], cold[0], cold[1], &br_, &bi_);                \
+        if (k <= 0 || (alpha[0] == (T)0 && alpha[1] == (T)0)) {                                \
+            out[0] = br_;                                                                      \
+            out[1] = bi_;                                                                      \
+            return;                                                                            \
+        }                                                                                      \
+        T sr = 0, si = 0;                                                                      \
+        for (int p = 0; p < k; ++p) {                                                          \
+            T xr, xi, yr, yi, pr, pi;                                                          \
+            NAME##_at(ta, a, lda, i, (size_t)p, &xr, &xi);                                     \
+            NAME##_at(tb, b, ldb, (size_t)p, j, &yr, &yi);                                     \
+            NAME##_mul(xr, xi, yr, yi, &pr, &pi);                                              \
+            sr = sr + pr;                                                                      \
+            si = si + pi;                                                                      \
+        }                                                                                      \
+        T ar_, ai_;                                                                            \
+        NAME##_mul(alpha[0], alpha[1], sr, si, &ar_, &ai_);                                    \
+        out[0] = ar_ + br_;                                                                    \
+        out[1] = ai_ + bi_;                                                                    \
+    }                                                                                          \
+    void NAME(char ta, char tb, int m, int n, int k, const T *alpha, const T *a, int lda,      \
+              const T *b, int ldb, const T *beta, T *c, int ldc) {                             \
+        if (m <= 0 || n <= 0) return;                                                          \
+        for (int j = 0; j < n; ++j)                                                            \
+            for (int i = 0; i < m; ++i) {                                                      \
+                T *cp = &c[2 * mi(ldc, i, j)];                                                 \
+                T out[2];                                                                      \
+                NAME##_one(ta, tb, k, alpha, a, lda, b, ldb, beta, cp, i, j, out);             \
+                cp[0] = out[0];                                                                \
+                cp[1] = out[1];                                                                \
+            }                                                                                  \
+    }                                                                                          \
+    /* the same value for a list of (row, col) entries of C; c0 may be NULL   */               \
+    /* when beta == 0; out holds interleaved (re, im) per entry               */               \
+    void ENTRIES(char ta, char tb, int k, const T *alpha, const T *a, int lda, const T *b,     \
+                 int ldb, const T *beta, const T *c0, int ldc, const int64_t *rows,            \
+                 const int64_t *cols, int64_t count, T *out) {                                 \
+        static const T zero2[2] = {0, 0};                                                      \
+        for (int64_t e = 0; e < count; ++e) {                                                  \
+            size_t i = (size_t)rows[e], j = (size_t)cols[e];                                   \
+            const T *cold = c0 ? &c0[2 * mi(ldc, i, j)] : zero2;                               \
+            NAME##_one(ta, tb, k, alpha, a, lda, b, ldb, beta, cold, i, j, &out[2 * e]);       \
+        }                                                                                      \
+    }
+
+DEFINE_CGEMM(oracle_zgemm, oracle_zgemm_entries, double)
+DEFINE_CGEMM(oracle_cgemm, oracle_cgemm_entries, float)
+
+/* dd value of alpha*sum_p op(A)(i,p) op(B)(p,j) + beta*C0(i,j), complex, for a
+   list of entries: out_hi / out_lo hold interleaved (re, im). */
+void oracle_zdd_gemm_entries(char ta, char tb, int k, const double *alpha, const double *a, int lda,
+                             const double *b, int ldb, const double *beta, const double *c0, int ldc,
+                             const int64_t *rows, const int64_t *cols, int64_t count, double *out_hi,
+                             double *out_lo) {
+    for (int64_t e = 0; e < count; ++e) {
+        size_t i = (size_t)rows[e], j = (size_t)cols[e];
+        dd sr = {0.0, 0.0}, si = {0.0, 0.0};
+        for (int p = 0; p < k; ++p) {
+            double xr, xi, yr, yi;
+            oracle_zgemm_at(ta, a, lda, i, (size_t)p, &xr, &xi);
+            oracle_zgemm_at(tb, b, ldb, (size_t)p, j, &yr, &yi);
+            sr = dd_add(sr, two_prod(xr, yr));
+            sr = dd_add(sr, two_prod(-xi, yi));
+            si = dd_add(si, two_prod(xr, yi));
+            si = dd_add(si, two_prod(xi, yr));
+        }
+        dd rr = dd_add(dd_mul_d(sr, alpha[0]), dd_mul_d(si, -alpha[1]));
+        dd ri = dd_add(dd_mul_d(si, alpha[0]), dd_mul_d(sr, alpha[1]));
+        if (beta[0] != 0.0 || beta[1] != 0.0) {
+            const double *cp = &c0[2 * mi(ldc, i, j)];
+            rr = dd_add(rr, dd_add(two_prod(beta[0], cp[0]), two_prod(-beta[1], cp[1])));
+            ri = dd_add(ri, dd_add(two_prod(beta[0], cp[1]), two_prod(beta[1], cp[0])));
+        }
+        out_hi[2 * e] = rr.hi;
+        out_lo[2 * e] = rr.lo;
+        out_hi[2 * e + 1] = ri.hi;
+        out_lo[2 * e + 1] = ri.lo;
     }
 }
 

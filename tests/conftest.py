@@ -129,3 +129,41 @@ def int_gemm(ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc):
     res = alpha * S.astype(np.float64) + (beta * old if beta != 0 else 0.0)
     view[:, :m] = res.T.astype(out.dtype)
     return out
+
+
+# ---------------------------------------------------------------------------
+# complex helpers (interleaved storage held as numpy complex arrays)
+# ---------------------------------------------------------------------------
+COMPLEX_SENTINEL = complex(-77, 33)  # fortran_compat_test.zig:476
+
+
+def fill_padded_complex(rows: int, cols: int, ld: int, salt: int, dtype=np.complex128, integer=False):
+    """fortran_compat_test.zig:175-194: re = sample(i, j, salt), im = sample(i, j, salt + 11),
+    the sentinel -77 + 33i outside the logical part."""
+    re = fill_padded(rows, cols, ld, salt, COMPLEX_SENTINEL.real, integer=integer)
+    im = fill_padded(rows, cols, ld, salt + 11, COMPLEX_SENTINEL.imag, integer=integer)
+    return (re + 1j * im).astype(dtype)
+
+
+def cmat(x: np.ndarray, rows: int, cols: int, ld: int) -> np.ndarray:
+    """Column-major rows x cols view (copy) of a flat complex buffer."""
+    return x[: ld * cols].reshape(cols, ld)[:, :rows].T.copy()
+
+
+def cop(x: np.ndarray, t: str) -> np.ndarray:
+    """op(X) for 'N', 'T', 'C'."""
+    return x if t == "N" else (x.T if t == "T" else x.conj().T)
+
+
+def int_cgemm(ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc):
+    """Exact complex result for integer-valued inputs and integer alpha/beta
+    (every partial sum is an integer far below 2^53)."""
+    A = cop(cmat(a, k if ta != "N" else m, m if ta != "N" else k, lda), ta)
+    B = cop(cmat(b, n if tb != "N" else k, k if tb != "N" else n, ldb), tb)
+    S = A.astype(np.complex128) @ B.astype(np.complex128)
+    out = c.copy()
+    view = out[: ldc * n].reshape(n, ldc)
+    old = view[:, :m].T.astype(np.complex128)
+    res = complex(alpha) * S + (complex(beta) * old if complex(beta) != 0 else 0)
+    view[:, :m] = res.T.astype(out.dtype)
+    return out

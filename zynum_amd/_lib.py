@@ -19,7 +19,7 @@ DECLINE_ARGUMENT = 4
 DECLINE_FORCED = 5
 ERR_LAUNCH = 6
 
-F32, F64 = 0, 1
+F32, F64, C32, C64 = 0, 1, 2, 3
 NO_TRANS, TRANS, CONJ_TRANS = 0, 1, 2
 MODE_PLAIN, MODE_OZAKI_INT8, MODE_OZAKI_BF16, MODE_OZAKI_CRT, MODE_DD = 0, 1, 2, 3, 4
 FLAG_ASYNC = 1
@@ -73,6 +73,12 @@ SIGNATURES = {
                            C.c_double, C.c_void_p, C.c_int]),
     "cblas_dtrsm": (None, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_void_p,
                            C.c_int, C.c_void_p, C.c_int]),
+    "cblas_cgemm": (None, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_int,
+                           C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_int]),
+    "cblas_zgemm": (None, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_int,
+                           C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_int]),
+    "cgemm_": (None, [_CH, _CH, _I, _I, _I, C.c_void_p, C.c_void_p, _I, C.c_void_p, _I, C.c_void_p, C.c_void_p, _I]),
+    "zgemm_": (None, [_CH, _CH, _I, _I, _I, C.c_void_p, C.c_void_p, _I, C.c_void_p, _I, C.c_void_p, C.c_void_p, _I]),
     "sgemm_": (None, [_CH, _CH, _I, _I, _I, _F, C.c_void_p, _I, C.c_void_p, _I, _F, C.c_void_p, _I]),
     "dgemm_": (None, [_CH, _CH, _I, _I, _I, _D, C.c_void_p, _I, C.c_void_p, _I, _D, C.c_void_p, _I]),
     "ssyrk_": (None, [_CH, _CH, _I, _I, _F, C.c_void_p, _I, _F, C.c_void_p, _I]),

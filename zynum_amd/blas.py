@@ -147,31 +147,48 @@ def _check_shapes(left: ConstMatrix, right: ConstMatrix, result: Matrix) -> None
         raise DimensionMismatch("result columns differ")
 
 
+_SCALARS = {"float64": _lib.F64, "float32": _lib.F32, "complex128": _lib.C64, "complex64": _lib.C32}
+
+
 def _scalar_kind(kind: str) -> int:
-    if kind == "float64":
-        return _lib.F64
-    if kind == "float32":
-        return _lib.F32
+    if kind in _SCALARS:
+        return _SCALARS[kind]
     raise Error(f"unsupported scalar type {kind}")
 
 
 def _bits(value: float, scalar: int) -> int:
-    if scalar == _lib.F64:
+    """Bit pattern of a real scalar (double, or float in the low 32 bits)."""
+    if scalar in (_lib.F64, _lib.C64):
         return struct.unpack("<Q", struct.pack("<d", float(value)))[0]
     return struct.unpack("<I", struct.pack("<f", float(value)))[0]
 
 
-def gemm_request(transa: bool, transb: bool, m: int, n: int, k: int, alpha: float, a: int, lda: int,
-                 b: int, ldb: int, beta: float, c: int, ldc: int, scalar: int, stream: int | None = None,
+def _trans_code(op) -> int:
+    """False / True / "N" / "T" / "C" (or a view operation name) to the request code."""
+    if op in (False, "N", "normal"):
+        return _lib.NO_TRANS
+    if op in (True, "T", "transposed"):
+        return _lib.TRANS
+    if op in ("C", "adjoint"):
+        return _lib.CONJ_TRANS
+    raise Error(f"unknown operation {op!r}")
+
+
+def gemm_request(transa, transb, m: int, n: int, k: int, alpha, a: int, lda: int,
+                 b: int, ldb: int, beta, c: int, ldc: int, scalar: int, stream: int | None = None,
                  mode: int = _lib.MODE_PLAIN, mode_param: int = 0, flags: int = 0, tile: int = 0) -> int:
-    """Fill and submit a ``zynum_gpu_gemm_request``; returns its status code."""
+    """Fill and submit a ``zynum_gpu_gemm_request``; returns its status code.
+
+    ``transa`` / ``transb``: bool or "N" / "T" / "C".  ``alpha`` / ``beta`` may
+    be complex for the complex scalar codes (C32 / C64)."""
     r = _lib.GemmRequest()
-    r.transa = _lib.TRANS if transa else _lib.NO_TRANS
-    r.transb = _lib.TRANS if transb else _lib.NO_TRANS
+    r.transa = _trans_code(transa)
+    r.transb = _trans_code(transb)
     r.scalar = scalar
     r.m, r.n, r.k, r.lda, r.ldb, r.ldc = m, n, k, lda, ldb, ldc
-    r.alpha_re = _bits(alpha, scalar)
-    r.beta_re = _bits(beta, scalar)
+    alpha, beta = complex(alpha), complex(beta)
+    r.alpha_re, r.alpha_im = _bits(alpha.real, scalar), _bits(alpha.imag, scalar)
+    r.beta_re, r.beta_im = _bits(beta.real, scalar), _bits(beta.imag, scalar)
     r.a, r.b, r.c = a, b, c
     r.stream = stream
     r.mode, r.mode_param, r.flags, r.tile = mode, mode_param, flags, tile
@@ -179,11 +196,11 @@ def gemm_request(transa: bool, transb: bool, m: int, n: int, k: int, alpha: floa
 
 
 def _run(left: ConstMatrix, right: ConstMatrix, out_addr: int, out_ld: int, rows: int, cols: int,
-         product_scale: float, result_scale: float, scalar: int, stream: int | None, mode: int,
+         product_scale, result_scale, scalar: int, stream: int | None, mode: int,
          mode_param: int) -> None:
     la, _, _, _ = _storage(left.values)
     ra, _, _, _ = _storage(right.values)
-    st = gemm_request(left.operation != "normal", right.operation != "normal", rows, cols,
+    st = gemm_request(left.operation, right.operation, rows, cols,
                       left.effective_column_count(), product_scale, la, max(1, left.leading_dimension), ra,
                       max(1, right.leading_dimension), result_scale, out_addr, max(1, out_ld), scalar, stream,
                       mode, mode_param)
@@ -191,7 +208,8 @@ def _run(left: ConstMatrix, right: ConstMatrix, out_addr: int, out_ld: int, rows
 
 
 def matrix_multiply(left_matrix: ConstMatrix, right_matrix: ConstMatrix, result_matrix: Matrix,
-                    product_scale: float = 1.0, result_scale: float = 0.0, *, stream: int | None = None,
+                    product_scale: float | complex = 1.0, result_scale: float | complex = 0.0, *,
+                    stream: int | None = None,
                     mode: int = _lib.MODE_PLAIN, mode_param: int = 0) -> None:
     """result = product_scale * op(left) * op(right) + result_scale * result (operations.zig:501-522)."""
     kinds = {_storage(x.values)[3] for x in (left_matrix, right_matrix, result_matrix)}

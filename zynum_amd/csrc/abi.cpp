@@ -146,6 +146,31 @@ int gemm_op(bool ta, bool tb, int m, int n, int k, T alpha, const T* a, int64_t 
 }
 
 template <typename T>
+int cgemm_op(char ta, char tb, int m, int n, int k, const T* alpha, const T* a, int64_t lda, const T* b,
+             int64_t ldb, const T* beta, T* c, int64_t ldc, hipStream_t s, int flags, int tile) {
+    if (m <= 0 || n <= 0) return ZYNUM_GPU_OK;  // gemm.zig:1393
+    if (take_forced_decline()) return ZYNUM_GPU_DECLINE_FORCED;
+    if (!device_ready()) return ZYNUM_GPU_DECLINE_NO_DEVICE;
+    const size_t es = 2 * sizeof(T);
+    const bool reads_ab = (k > 0) && !(alpha[0] == T(0) && alpha[1] == T(0));
+    const bool beta_zero = beta[0] == T(0) && beta[1] == T(0);
+    Operand oa, ob, oc;
+    int st = ZYNUM_GPU_OK;
+    if (reads_ab) {
+        st = oa.bind_in(a, ta != 'N' ? k : m, ta != 'N' ? m : k, lda, es, s);
+        if (st == ZYNUM_GPU_OK) st = ob.bind_in(b, tb != 'N' ? n : k, tb != 'N' ? k : n, ldb, es, s);
+    }
+    if (st == ZYNUM_GPU_OK) st = oc.bind_out(c, m, n, ldc, es, !beta_zero, s);
+    if (st != ZYNUM_GPU_OK) return st;
+    st = cgemm_device<T>(ta, tb, m, n, reads_ab ? k : 0, alpha[0], alpha[1], static_cast<const T*>(oa.dev), lda,
+                         static_cast<const T*>(ob.dev), ldb, beta[0], beta[1], static_cast<T*>(oc.dev), ldc, s, tile);
+    if (st != ZYNUM_GPU_OK) return st;
+    if (oc.staged) return oc.finish(s);
+    if (!(flags & ZYNUM_GPU_FLAG_ASYNC) || oa.staged || ob.staged) return status_from_hip(hipStreamSynchronize(s));
+    return ZYNUM_GPU_OK;
+}
+
+template <typename T>
 int trsm_op(char side, char uplo, bool trans, char diag, int m, int n, T alpha, const T* a, int64_t lda, T* b,
             int64_t ldb, hipStream_t s, int mode, int mode_param, int flags) {
     if (m <= 0 || n <= 0) return ZYNUM_GPU_OK;  // triangular.zig:234
@@ -190,6 +215,10 @@ template int gemm_op<double>(bool, bool, int, int, int, double, const double*, i
                              double, double*, int64_t, hipStream_t, int, int, int, int);
 template int gemm_op<float>(bool, bool, int, int, int, float, const float*, int64_t, const float*, int64_t, float,
                             float*, int64_t, hipStream_t, int, int, int, int);
+template int cgemm_op<double>(char, char, int, int, int, const double*, const double*, int64_t, const double*,
+                              int64_t, const double*, double*, int64_t, hipStream_t, int, int);
+template int cgemm_op<float>(char, char, int, int, int, const float*, const float*, int64_t, const float*, int64_t,
+                             const float*, float*, int64_t, hipStream_t, int, int);
 
 }  // namespace zg
 
@@ -236,6 +265,29 @@ void dgemm_(const char* ta, const char* tb, const zynum_blas_int* m, const zynum
     if (report_error("DGEMM ", gemm_error(A, B, *m, *n, *k, *lda, *ldb, *ldc))) return;
     note_status("DGEMM", zg::gemm_op<double>(A != 'N', B != 'N', *m, *n, *k, *alpha, a, *lda, b, *ldb, *beta, c,
                                              *ldc, zg::blas_stream(), ZYNUM_GPU_MODE_PLAIN, 0, 0));
+}
+
+// fortran.zig:768-775: 'C' is the conjugate transpose for the complex forms.
+void cgemm_(const char* ta, const char* tb, const zynum_blas_int* m, const zynum_blas_int* n, const zynum_blas_int* k,
+            const zynum_blas_complex_float* alpha, const zynum_blas_complex_float* a, const zynum_blas_int* lda,
+            const zynum_blas_complex_float* b, const zynum_blas_int* ldb, const zynum_blas_complex_float* beta,
+            zynum_blas_complex_float* c, const zynum_blas_int* ldc) {
+    const char A = up(ta), B = up(tb);
+    if (report_error("CGEMM ", gemm_error(A, B, *m, *n, *k, *lda, *ldb, *ldc))) return;
+    note_status("CGEMM", zg::cgemm_op<float>(A, B, *m, *n, *k, &alpha->real, reinterpret_cast<const float*>(a), *lda,
+                                             reinterpret_cast<const float*>(b), *ldb, &beta->real,
+                                             reinterpret_cast<float*>(c), *ldc, zg::blas_stream(), 0));
+}
+
+void zgemm_(const char* ta, const char* tb, const zynum_blas_int* m, const zynum_blas_int* n, const zynum_blas_int* k,
+            const zynum_blas_complex_double* alpha, const zynum_blas_complex_double* a, const zynum_blas_int* lda,
+            const zynum_blas_complex_double* b, const zynum_blas_int* ldb, const zynum_blas_complex_double* beta,
+            zynum_blas_complex_double* c, const zynum_blas_int* ldc) {
+    const char A = up(ta), B = up(tb);
+    if (report_error("ZGEMM ", gemm_error(A, B, *m, *n, *k, *lda, *ldb, *ldc))) return;
+    note_status("ZGEMM", zg::cgemm_op<double>(A, B, *m, *n, *k, &alpha->real, reinterpret_cast<const double*>(a), *lda,
+                                              reinterpret_cast<const double*>(b), *ldb, &beta->real,
+                                              reinterpret_cast<double*>(c), *ldc, zg::blas_stream(), 0));
 }
 
 void ssyrk_(const char* u, const char* t, const zynum_blas_int* n, const zynum_blas_int* k, const float* alpha,
@@ -308,6 +360,30 @@ void cblas_dgemm(int layout, int transa, int transb, int m, int n, int k, double
     dgemm_(&ta, &tb, &mm, &nn, &kk, &alpha, row ? b : a, &la, row ? a : b, &lb, &beta, c, &lc);
 }
 
+// cblas.zig:1441-1465: row-major swaps the operands like the real forms; the
+// transpose codes keep their meaning (a conjugate transpose stays one).
+void cblas_cgemm(int layout, int transa, int transb, int m, int n, int k, const void* alpha, const void* a, int lda,
+                 const void* b, int ldb, const void* beta, void* c, int ldc) {
+    if (!cb_layout(layout) || !cb_trans(transa) || !cb_trans(transb)) return;
+    const bool row = layout == CblasRowMajor;
+    char ta = cb_trans_char(row ? transb : transa), tb = cb_trans_char(row ? transa : transb);
+    zynum_blas_int mm = row ? n : m, nn = row ? m : n, kk = k, la = row ? ldb : lda, lb = row ? lda : ldb, lc = ldc;
+    typedef zynum_blas_complex_float cf;
+    cgemm_(&ta, &tb, &mm, &nn, &kk, static_cast<const cf*>(alpha), static_cast<const cf*>(row ? b : a), &la,
+           static_cast<const cf*>(row ? a : b), &lb, static_cast<const cf*>(beta), static_cast<cf*>(c), &lc);
+}
+
+void cblas_zgemm(int layout, int transa, int transb, int m, int n, int k, const void* alpha, const void* a, int lda,
+                 const void* b, int ldb, const void* beta, void* c, int ldc) {
+    if (!cb_layout(layout) || !cb_trans(transa) || !cb_trans(transb)) return;
+    const bool row = layout == CblasRowMajor;
+    char ta = cb_trans_char(row ? transb : transa), tb = cb_trans_char(row ? transa : transb);
+    zynum_blas_int mm = row ? n : m, nn = row ? m : n, kk = k, la = row ? ldb : lda, lb = row ? lda : ldb, lc = ldc;
+    typedef zynum_blas_complex_double cd;
+    zgemm_(&ta, &tb, &mm, &nn, &kk, static_cast<const cd*>(alpha), static_cast<const cd*>(row ? b : a), &la,
+           static_cast<const cd*>(row ? a : b), &lb, static_cast<const cd*>(beta), static_cast<cd*>(c), &lc);
+}
+
 void cblas_dsyrk(int layout, int uplo, int trans, int n, int k, double alpha, const double* a, int lda, double beta,
                  double* c, int ldc) {
     if (!cb_layout(layout)) return;
@@ -337,15 +413,33 @@ void cblas_dtrsm(int layout, int side, int uplo, int transa, int diag, int m, in
 // ---------------------------------------------------------------------------
 int zynum_gpu_gemm(const zynum_gpu_gemm_request* r) {
     if (r == nullptr) return ZYNUM_GPU_DECLINE_ARGUMENT;
-    const char ta = r->transa == ZYNUM_GPU_NO_TRANS ? 'N' : r->transa <= ZYNUM_GPU_CONJ_TRANS ? 'T' : 'X';
-    const char tb = r->transb == ZYNUM_GPU_NO_TRANS ? 'N' : r->transb <= ZYNUM_GPU_CONJ_TRANS ? 'T' : 'X';
+    const bool cplx = r->scalar == ZYNUM_GPU_C32 || r->scalar == ZYNUM_GPU_C64;
+    auto code = [cplx](uint8_t t) {
+        return t == ZYNUM_GPU_NO_TRANS ? 'N' : t == ZYNUM_GPU_TRANS ? 'T' : t == ZYNUM_GPU_CONJ_TRANS ? (cplx ? 'C' : 'T') : 'X';
+    };
+    const char ta = code(r->transa), tb = code(r->transb);
     if (gemm_error(ta, tb, r->m, r->n, r->k, r->lda, r->ldb, r->ldc) != 0) return ZYNUM_GPU_DECLINE_ARGUMENT;
-    if (r->alpha_im != 0 || r->beta_im != 0) return ZYNUM_GPU_DECLINE_UNSUPPORTED;
     const int tiles = r->mode == ZYNUM_GPU_MODE_PLAIN       ? zg::kTileConfigs
                       : r->mode == ZYNUM_GPU_MODE_OZAKI_BF16 ? zg::kBf16Tiles
                                                              : zg::kI8Tiles;
     if (r->tile > tiles) return ZYNUM_GPU_DECLINE_UNSUPPORTED;
     hipStream_t s = r->stream ? reinterpret_cast<hipStream_t>(r->stream) : zg::blas_stream();
+    if (cplx) {
+        if (r->mode != ZYNUM_GPU_MODE_PLAIN) return ZYNUM_GPU_DECLINE_UNSUPPORTED;
+        if (r->scalar == ZYNUM_GPU_C64) {
+            const double alpha[2] = {scalar_from_bits<double>(r->alpha_re), scalar_from_bits<double>(r->alpha_im)};
+            const double beta[2] = {scalar_from_bits<double>(r->beta_re), scalar_from_bits<double>(r->beta_im)};
+            return zg::cgemm_op<double>(ta, tb, r->m, r->n, r->k, alpha, static_cast<const double*>(r->a), r->lda,
+                                        static_cast<const double*>(r->b), r->ldb, beta, static_cast<double*>(r->c),
+                                        r->ldc, s, r->flags, (int)r->tile - 1);
+        }
+        const float alpha[2] = {scalar_from_bits<float>(r->alpha_re), scalar_from_bits<float>(r->alpha_im)};
+        const float beta[2] = {scalar_from_bits<float>(r->beta_re), scalar_from_bits<float>(r->beta_im)};
+        return zg::cgemm_op<float>(ta, tb, r->m, r->n, r->k, alpha, static_cast<const float*>(r->a), r->lda,
+                                   static_cast<const float*>(r->b), r->ldb, beta, static_cast<float*>(r->c), r->ldc,
+                                   s, r->flags, (int)r->tile - 1);
+    }
+    if (r->alpha_im != 0 || r->beta_im != 0) return ZYNUM_GPU_DECLINE_UNSUPPORTED;
     if (r->scalar == ZYNUM_GPU_F64) {
         return zg::gemm_op<double>(ta == 'T', tb == 'T', r->m, r->n, r->k, scalar_from_bits<double>(r->alpha_re),
                                    static_cast<const double*>(r->a), r->lda, static_cast<const double*>(r->b), r->ldb,

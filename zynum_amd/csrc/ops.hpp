@@ -20,7 +20,16 @@ template <typename T>
 int syrk_op(char uplo, bool trans, int n, int k, T alpha, const T* a, int64_t lda, T beta, T* c, int64_t ldc,
             hipStream_t s, int mode, int mode_param, int flags);
 
+// Complex GEMM; a, b, c point at interleaved (re, im) storage, leading
+// dimensions and alpha/beta pointers are in complex elements.  ta/tb: 'N', 'T', 'C'.
+template <typename T>
+int cgemm_op(char ta, char tb, int m, int n, int k, const T* alpha, const T* a, int64_t lda, const T* b,
+             int64_t ldb, const T* beta, T* c, int64_t ldc, hipStream_t s, int flags, int tile = -1);
+
 // Device-pointer implementations (no staging), stream ordered.
+template <typename T>
+int cgemm_device(char ta, char tb, int m, int n, int k, T alpha_re, T alpha_im, const T* a, int64_t lda, const T* b,
+                 int64_t ldb, T beta_re, T beta_im, T* c, int64_t ldc, hipStream_t s, int tile);
 int ozaki_dgemm(bool ta, bool tb, int m, int n, int k, double alpha, const double* a, int64_t lda,
                 const double* b, int64_t ldb, double beta, double* c, int64_t ldc, hipStream_t s, int mode,
                 int mode_param, int tile = -1);
