@@ -22,6 +22,11 @@
 #include "gemm.hpp"
 #include "gemm_tile.hpp"
 
+// Tile rows per super-row of the grouped tile order; a build knob for tuning.
+#ifndef ZG_GROUP_M
+#define ZG_GROUP_M 8
+#endif
+
 namespace zg {
 
 template <typename T, class CF, bool TA, bool TB, int VEC>
@@ -67,7 +72,7 @@ static hipError_t launch_cfg(const GemmDesc<T>& d, hipStream_t stream) {
     p.a = d.a; p.b = d.b; p.c = d.c;
     p.tiles_m = (d.m + CF::BM - 1) / CF::BM;
     p.tiles_n = (d.n + CF::BN - 1) / CF::BN;
-    p.group_m = 8;
+    p.group_m = ZG_GROUP_M;
     const int grid = p.tiles_m * p.tiles_n;
     if (!d.ta && !d.tb) gemm_mfma_kernel<T, CF, false, false, VEC><<<grid, CF::NT, 0, stream>>>(p);
     else if (!d.ta && d.tb) gemm_mfma_kernel<T, CF, false, true, VEC><<<grid, CF::NT, 0, stream>>>(p);

@@ -136,11 +136,10 @@ __device__ __forceinline__ void gemm_tile(const GemmArgs<T>& p, int m0, int n0, 
     const int wm0 = (wave % CF::WM) * WTM;
     const int wn0 = (wave / CF::WM) * WTN;
 
-
     // op(A) tile rows are M (A: M-contiguous unless TA); op(B) rows are N
     // (B^T: N-contiguous when TB, otherwise K-contiguous).
-    Stage<T, BM, BK, TA, VEC, NT, PAD> sa;
-    Stage<T, BN, BK, !TB, VEC, NT, PAD> sb;
+    using StA = Stage<T, BM, BK, TA, VEC, NT, PAD>;
+    using StB = Stage<T, BN, BK, !TB, VEC, NT, PAD>;
 
     Acc acc[FM][FN];
 #pragma unroll
@@ -151,27 +150,23 @@ __device__ __forceinline__ void gemm_tile(const GemmArgs<T>& p, int m0, int n0, 
     const int nk = (p.k + BK - 1) / BK;
     const bool mn_full = (m0 + BM <= p.m) && (n0 + BN <= p.n);
 
-    sa.load(p.a, p.lda, m0, 0, p.m, p.k, mn_full && BK <= p.k, tid);
-    sb.load(p.b, p.ldb, n0, 0, p.n, p.k, mn_full && BK <= p.k, tid);
-    sa.store(lds_a, tid);
-    sb.store(lds_b, tid);
-    __syncthreads();
-
     const int lr = lane & 15;  // column within a 16x16 block (B, D)
     const int lk = lane >> 4;  // k within a 4-deep step
     const int ar = M::a_row(lr);  // A row fed by this lane
 
-    for (int t = 0; t < nk; ++t) {
-        const int cur = t & 1;
-        const bool more = (t + 1) < nk;
-        if (more) {
-            const int k0 = (t + 1) * BK;
-            const bool full = mn_full && (k0 + BK <= p.k);
-            sa.load(p.a, p.lda, m0, k0, p.m, p.k, full, tid);
-            sb.load(p.b, p.ldb, n0, k0, p.n, p.k, full, tid);
-        }
-        const T* __restrict__ la = lds_a + cur * (BK * (BM + PAD));
-        const T* __restrict__ lb = lds_b + cur * (BK * (BN + PAD));
+    auto load = [&](StA& sa, StB& sb, int t) {
+        const int k0 = t * BK;
+        const bool full = mn_full && (k0 + BK <= p.k);
+        sa.load(p.a, p.lda, m0, k0, p.m, p.k, full, tid);
+        sb.load(p.b, p.ldb, n0, k0, p.n, p.k, full, tid);
+    };
+    auto store = [&](const StA& sa, const StB& sb, int buf) {
+        sa.store(lds_a + buf * (BK * (BM + PAD)), tid);
+        sb.store(lds_b + buf * (BK * (BN + PAD)), tid);
+    };
+    auto multiply = [&](int buf) {
+        const T* __restrict__ la = lds_a + buf * (BK * (BM + PAD));
+        const T* __restrict__ lb = lds_b + buf * (BK * (BN + PAD));
 #pragma unroll
         for (int kk = 0; kk < BK; kk += 4) {
             T af[FM], bf[FN];
@@ -184,10 +179,18 @@ __device__ __forceinline__ void gemm_tile(const GemmArgs<T>& p, int m0, int n0, 
 #pragma unroll
                 for (int j = 0; j < FN; ++j) acc[i][j] = M::mma(af[i], bf[j], acc[i][j]);
         }
-        if (more) {
-            sa.store(lds_a + (cur ^ 1) * (BK * (BM + PAD)), tid);
-            sb.store(lds_b + (cur ^ 1) * (BK * (BN + PAD)), tid);
-        }
+    };
+
+    StA sa;
+    StB sb;
+    load(sa, sb, 0);
+    store(sa, sb, 0);
+    __syncthreads();
+    for (int t = 0; t < nk; ++t) {
+        const bool more = (t + 1) < nk;
+        if (more) load(sa, sb, t + 1);
+        multiply(t & 1);
+        if (more) store(sa, sb, (t + 1) & 1);
         __syncthreads();
     }
 

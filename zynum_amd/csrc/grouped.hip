@@ -99,6 +99,7 @@ __global__ __launch_bounds__(CF::NT) __attribute__((amdgpu_waves_per_eu(CF::OCC)
     auto barrier = [&]() {  // LDS reads and writes of this step done; global loads may stay in flight
         asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
         __builtin_amdgcn_s_barrier();
+        asm volatile("" ::: "memory");  // no LDS access moves above the barrier
     };
 
     // Descriptors are uniform (scalar loads); the next one is read only where
