@@ -290,7 +290,7 @@ def split_leg(torch, lib, A, B, n, ld, stream, plain_tf):
     out = {"workload": f"DGEMM {n}^3 NN, a_ij = u*2^e (u uniform [-1,1), e uniform in [-16,16]), seed 7",
            "plain_fp64": plain}
     best = None
-    for moduli in (14, 15, 16):
+    for moduli in (13, 14, 15, 16):
         r = run(_lib.MODE_OZAKI_CRT, moduli, 5)
         r["int8_gemms"] = moduli
         r["int8_tops_executed"] = round(moduli * flop / (r["ms"] * 1e-3) / 1e12, 1)
