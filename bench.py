@@ -432,7 +432,95 @@ def level3_leg(torch, lib, n, stream):
                     "max_abs_err_vs_dd": float(err)}
         del C
     out["dsyrk"] = res
+    del A, ref_hi, ref_lo
+    out.update(level3_more(torch, lib, n, stream))
     log(f"level3: {out}")
+    return out
+
+
+def level3_more(torch, lib, n, stream, S=1024):
+    """DSYMM Left/Lower M=N=n, DTRMM Left/Lower/NoTrans/NonUnit M=N=n and DSYR2K
+    Lower/NoTrans N=n, K=n/2 (beta 0), timed over 3 calls each; errors on an
+    S x S corner against the device double-double product of the expanded
+    operands (torch builds the expansions for the check only)."""
+    from zynum_amd import _lib, synth
+
+    dev = torch.device("cuda:0")
+    sh = stream.cuda_stream
+    out = {}
+
+    def timed(fn, reps=3):
+        fn()
+        torch.cuda.synchronize()
+        s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        s.record(stream)
+        for _ in range(reps):
+            fn()
+        e.record(stream)
+        torch.cuda.synchronize()
+        return s.elapsed_time(e) / reps
+
+    def dd_corner(x, ldx, y, ldy, k):
+        """dd of X[:S, :k] Y[:k, :S] (column-major device buffers) as (hi, lo) S x S."""
+        hi = torch.zeros(S * S, dtype=torch.float64, device=dev)
+        lo = torch.zeros(S * S, dtype=torch.float64, device=dev)
+        _lib.check(lib.zynum_gpu_dd_gemm(b"N", b"N", S, S, k, 1.0, x.data_ptr(), ldx, y.data_ptr(), ldy, 0.0, None, S,
+                                         hi.data_ptr(), lo.data_ptr(), S, sh), "dd reference")
+        return hi.view(S, S), lo.view(S, S)  # [j, i]
+
+    def err_of(got_cm, ld, hi, lo):
+        g = got_cm.view(-1, ld)[:S, :S]
+        e = ((g - hi) - lo).abs()
+        return float(e.max()), float(e.max() / hi.abs().max())
+
+    A = torch.from_numpy(synth.matrix(13, 0, n, n, n)).to(dev)           # column-major n x n
+    B = torch.from_numpy(synth.matrix(13, n * n, n, n, n)).to(dev)
+    # --- DSYMM: S from the lower triangle of A ---
+    C = torch.zeros(n * n, dtype=torch.float64, device=dev)
+    ms = timed(lambda: _lib.check(lib.zynum_gpu_dsymm(b"L", b"L", n, n, 1.0, A.data_ptr(), n, B.data_ptr(), n, 0.0,
+                                                      C.data_ptr(), n, sh, _lib.FLAG_ASYNC), "dsymm"))
+    At = A.view(n, n)                        # [j, i] = A(i, j)
+    low = torch.triu(At)                     # keeps A(i, j) with i >= j
+    Sfull = (low + low.T - torch.diag(torch.diagonal(At))).contiguous()   # symmetric: either orientation
+    hi, lo = dd_corner(Sfull, n, B, n, n)
+    e, ne = err_of(C, n, hi, lo)
+    out["dsymm"] = {"workload": f"DSYMM Left Lower M=N={n}, alpha 1, beta 0", "ms": round(ms, 3),
+                    "tflops": round(2.0 * n ** 3 / (ms * 1e-3) / 1e12, 2),
+                    "max_abs_err_vs_dd_corner": e, "normwise_err_vs_dd_corner": ne}
+    del Sfull, C, hi, lo
+    # --- DTRMM: lower triangle of A, B in place (a fresh copy per call) ---
+    X = B.clone()
+    flop = float(n) * n * n  # m^2 n
+    ms_total = timed(lambda: (X.copy_(B), _lib.check(lib.zynum_gpu_dtrmm(b"L", b"L", b"N", b"N", n, n, 1.0,
+                                                                          A.data_ptr(), n, X.data_ptr(), n, sh,
+                                                                          _lib.FLAG_ASYNC), "dtrmm")))
+    Tl = torch.triu(At).contiguous()        # [j, i] = A(i, j) for i >= j: the lower triangle, column-major
+    hi, lo = dd_corner(Tl, n, B, n, n)
+    e, ne = err_of(X, n, hi, lo)            # X holds the last call's result
+    ms_copy = timed(lambda: X.copy_(B))
+    ms = ms_total - ms_copy
+    out["dtrmm"] = {"workload": f"DTRMM Left Lower NoTrans NonUnit M=N={n}, alpha 1", "ms": round(ms, 3),
+                    "tflops": round(flop / (ms * 1e-3) / 1e12, 2),
+                    "max_abs_err_vs_dd_corner": e, "normwise_err_vs_dd_corner": ne}
+    del X, Tl, hi, lo
+    # --- DSYR2K: C_lower = A B^T + B A^T, A and B n x k ---
+    k = n // 2
+    C = torch.zeros(n * n, dtype=torch.float64, device=dev)
+    ms = timed(lambda: _lib.check(lib.zynum_gpu_dsyr2k(b"L", b"N", n, k, 1.0, A.data_ptr(), n, B.data_ptr(), n, 0.0,
+                                                       C.data_ptr(), n, sh, _lib.FLAG_ASYNC), "dsyr2k"))
+    Bt = B.view(n, n)[:k, :].T.contiguous()            # B(:, :k)^T as column-major k x n
+    At2 = A.view(n, n)[:k, :].T.contiguous()
+    h1, l1 = dd_corner(A, n, Bt, k, k)
+    h2, l2 = dd_corner(B, n, At2, k, k)
+    hi, lo = h1 + h2, l1 + l2
+    g = C.view(n, n)[:S, :S]
+    lower_mask = torch.triu(torch.ones(S, S, dtype=torch.bool, device=dev))  # [j, i] with i >= j
+    e_t = ((g - hi) - lo).abs()[lower_mask]
+    out["dsyr2k"] = {"workload": f"DSYR2K Lower NoTrans N={n} K={k}, alpha 1, beta 0", "ms": round(ms, 3),
+                     "tflops": round(2.0 * n * n * k / (ms * 1e-3) / 1e12, 2),
+                     "max_abs_err_vs_dd_corner": float(e_t.max()),
+                     "normwise_err_vs_dd_corner": float(e_t.max() / hi.abs()[lower_mask].max())}
+    del A, B, C, Bt, At2
     return out
 
 

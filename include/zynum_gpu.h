@@ -24,6 +24,11 @@
  *   dtrsm_ / cblas_dtrsm       src/blas/abi/fortran.zig:873-876,
  *                              src/blas/abi/cblas.zig:1742-1753, cblas.h:1755
  *   ssyrk_ / strsm_            src/blas/abi/fortran.zig:802-805,869-872
+ *   ssymm_ / dsymm_            src/blas/abi/fortran.zig:777-784 (checks :239-250)
+ *   ssyr2k_ / dsyr2k_          src/blas/abi/fortran.zig:827-834 (checks :264-275)
+ *   strmm_ / dtrmm_            src/blas/abi/fortran.zig:852-859 (checks :105-117)
+ *   cblas_{s,d}symm, cblas_{s,d}syr2k, cblas_{s,d}trmm
+ *                              src/blas/abi/cblas.zig:1467-1489,1605-1627,1677-1700
  *   xerbla_ / lsame_           src/blas/abi/fortran.zig:16-22
  *   zynum_blas_shutdown(_)     src/blas/abi/fortran.zig:41-47
  *   zynum_gpu_gemm_request     modeled on GemmRequest,
@@ -78,6 +83,18 @@ void cblas_cgemm(int layout, int transa, int transb, int m, int n, int k, const 
                  const void *a, int lda, const void *b, int ldb, const void *beta, void *c, int ldc);
 void cblas_zgemm(int layout, int transa, int transb, int m, int n, int k, const void *alpha,
                  const void *a, int lda, const void *b, int ldb, const void *beta, void *c, int ldc);
+void cblas_ssymm(int layout, int side, int uplo, int m, int n, float alpha, const float *a, int lda,
+                 const float *b, int ldb, float beta, float *c, int ldc);
+void cblas_dsymm(int layout, int side, int uplo, int m, int n, double alpha, const double *a, int lda,
+                 const double *b, int ldb, double beta, double *c, int ldc);
+void cblas_ssyr2k(int layout, int uplo, int trans, int n, int k, float alpha, const float *a, int lda,
+                  const float *b, int ldb, float beta, float *c, int ldc);
+void cblas_dsyr2k(int layout, int uplo, int trans, int n, int k, double alpha, const double *a, int lda,
+                  const double *b, int ldb, double beta, double *c, int ldc);
+void cblas_strmm(int layout, int side, int uplo, int transa, int diag, int m, int n, float alpha,
+                 const float *a, int lda, float *b, int ldb);
+void cblas_dtrmm(int layout, int side, int uplo, int transa, int diag, int m, int n, double alpha,
+                 const double *a, int lda, double *b, int ldb);
 void cblas_dsyrk(int layout, int uplo, int trans, int n, int k, double alpha, const double *a,
                  int lda, double beta, double *c, int ldc);
 void cblas_dtrsm(int layout, int side, int uplo, int transa, int diag, int m, int n, double alpha,
@@ -109,6 +126,24 @@ void ssyrk_(const char *uplo, const char *trans, const zynum_blas_int *n, const 
 void dsyrk_(const char *uplo, const char *trans, const zynum_blas_int *n, const zynum_blas_int *k,
             const double *alpha, const double *a, const zynum_blas_int *lda, const double *beta,
             double *c, const zynum_blas_int *ldc);
+void ssymm_(const char *side, const char *uplo, const zynum_blas_int *m, const zynum_blas_int *n,
+            const float *alpha, const float *a, const zynum_blas_int *lda, const float *b,
+            const zynum_blas_int *ldb, const float *beta, float *c, const zynum_blas_int *ldc);
+void dsymm_(const char *side, const char *uplo, const zynum_blas_int *m, const zynum_blas_int *n,
+            const double *alpha, const double *a, const zynum_blas_int *lda, const double *b,
+            const zynum_blas_int *ldb, const double *beta, double *c, const zynum_blas_int *ldc);
+void ssyr2k_(const char *uplo, const char *trans, const zynum_blas_int *n, const zynum_blas_int *k,
+             const float *alpha, const float *a, const zynum_blas_int *lda, const float *b,
+             const zynum_blas_int *ldb, const float *beta, float *c, const zynum_blas_int *ldc);
+void dsyr2k_(const char *uplo, const char *trans, const zynum_blas_int *n, const zynum_blas_int *k,
+             const double *alpha, const double *a, const zynum_blas_int *lda, const double *b,
+             const zynum_blas_int *ldb, const double *beta, double *c, const zynum_blas_int *ldc);
+void strmm_(const char *side, const char *uplo, const char *transa, const char *diag,
+            const zynum_blas_int *m, const zynum_blas_int *n, const float *alpha, const float *a,
+            const zynum_blas_int *lda, float *b, const zynum_blas_int *ldb);
+void dtrmm_(const char *side, const char *uplo, const char *transa, const char *diag,
+            const zynum_blas_int *m, const zynum_blas_int *n, const double *alpha, const double *a,
+            const zynum_blas_int *lda, double *b, const zynum_blas_int *ldb);
 void strsm_(const char *side, const char *uplo, const char *transa, const char *diag,
             const zynum_blas_int *m, const zynum_blas_int *n, const float *alpha, const float *a,
             const zynum_blas_int *lda, float *b, const zynum_blas_int *ldb);
@@ -232,6 +267,18 @@ int zynum_gpu_dtrsm(char side, char uplo, char transa, char diag, int m, int n, 
 int zynum_gpu_dsyrk(char uplo, char trans, int n, int k, double alpha, const double *a, int lda,
                     double beta, double *c, int ldc, void *stream, int mode, int mode_param,
                     int flags);
+
+/* DSYMM, DTRMM and DSYR2K on the GEMM core with a caller stream (arguments follow
+   dsymm_ / dtrmm_ / dsyr2k_, symmetricMatrixError / triangularMatrixError /
+   rank2KMatrixError).  SYMM expands the symmetric operand and TRMM the
+   triangle into the library workspace (synchronous whatever the flags);
+   SYR2K allocates nothing. */
+int zynum_gpu_dsymm(char side, char uplo, int m, int n, double alpha, const double *a, int lda,
+                    const double *b, int ldb, double beta, double *c, int ldc, void *stream, int flags);
+int zynum_gpu_dtrmm(char side, char uplo, char transa, char diag, int m, int n, double alpha,
+                    const double *a, int lda, double *b, int ldb, void *stream, int flags);
+int zynum_gpu_dsyr2k(char uplo, char trans, int n, int k, double alpha, const double *a, int lda,
+                     const double *b, int ldb, double beta, double *c, int ldc, void *stream, int flags);
 
 /* Stream used by the Part 1 symbols (default: the null stream). */
 void zynum_gpu_set_stream(void *stream);

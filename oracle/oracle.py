@@ -155,3 +155,29 @@ def zdd_gemm_entries(ta, tb, k, alpha, a, lda, b, ldb, beta, c0, ldc, rows, cols
     _cfuncs().oracle_zdd_gemm_entries(_c(ta), _c(tb), k, _p(al), _p(a), lda, _p(b), ldb, _p(be), _p(c0), ldc,
                                       _p(rows), _p(cols), rows.size, _p(hi), _p(lo))
     return hi, lo
+
+
+def _l3funcs():
+    lib = load()
+    if not getattr(lib, "_l3_bound", False):
+        vp, i, d, ch = C.c_void_p, C.c_int, C.c_double, C.c_char
+        lib.oracle_dsymm.argtypes = [ch, ch, i, i, d, vp, i, vp, i, d, vp, i]
+        lib.oracle_dsyr2k.argtypes = [ch, ch, i, i, d, vp, i, vp, i, d, vp, i]
+        lib.oracle_dtrmm.argtypes = [ch, ch, ch, ch, i, i, d, vp, i, vp, i]
+        lib._l3_bound = True
+    return lib
+
+
+def dsymm(side, uplo, m, n, alpha, a, lda, b, ldb, beta, c, ldc) -> None:
+    """In-place on ``c`` (symmetric.zig:56-70)."""
+    _l3funcs().oracle_dsymm(_c(side), _c(uplo), m, n, alpha, _p(a), lda, _p(b), ldb, beta, _p(c), ldc)
+
+
+def dsyr2k(uplo, trans, n, k, alpha, a, lda, b, ldb, beta, c, ldc) -> None:
+    """In-place on the stored triangle of ``c`` (symmetric.zig:301-329)."""
+    _l3funcs().oracle_dsyr2k(_c(uplo), _c(trans), n, k, alpha, _p(a), lda, _p(b), ldb, beta, _p(c), ldc)
+
+
+def dtrmm(side, uplo, trans, diag, m, n, alpha, a, lda, b, ldb) -> None:
+    """In-place on ``b`` (triangular.zig:202-231)."""
+    _l3funcs().oracle_dtrmm(_c(side), _c(uplo), _c(trans), _c(diag), m, n, alpha, _p(a), lda, _p(b), ldb)

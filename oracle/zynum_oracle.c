@@ -2,7 +2,8 @@
  * TEST INFRASTRUCTURE ONLY -- the oracle.
  *
  * A plain-C restatement of Zynum's portable CPU Level-3 path (GEMM, left/right
- * TRSM, SYRK) plus a double-double GEMM used to state errors.  Only the test
+ * TRSM, SYRK, SYMM, SYR2K, TRMM, complex GEMM) plus double-double GEMMs used
+ * to state errors.  Only the test
  * suite, the smoke check and the benchmark's `cpu_baseline` leg load this
  * library, and only as the checker or as the timed CPU baseline.  Nothing in
  * the zynum_amd product links or calls it.
@@ -21,6 +22,8 @@
  *   - TRSM: core/matrix_matrix/triangular.zig:52-63,72-84,103-118,233-249 and
  *           core/matrix_vector/triangular.zig:44-57,362-412
  *   - SYRK: core/matrix_matrix/symmetric.zig:180-200,237-279
+ *   - SYMM, SYR2K: symmetric.zig:56-160,301-412; TRMM: triangular.zig:202-231
+ *           with matrix_vector/triangular.zig:44-133 (see below)
  *   - complex GEMM (CGEMM / ZGEMM): gemm.zig:1234-1246,1404-1418 with the
  *           complex scalar rules of core/shared/scalar.zig (see below)
  *
@@ -525,6 +528,146 @@ void oracle_dsyrk(char uplo, char trans, int n, int k, double alpha, const doubl
     syrk_task t = {(uplo == 'U' || uplo == 'u'), !is_no_trans(trans), (size_t)n, (size_t)(k < 0 ? 0 : k),
                    alpha, beta, a, lda, c, ldc};
     run_column_tasks(syrk_cols, &t, (size_t)n, 1);
+}
+
+/* ------------------------------------------------------------------------ */
+/* SYMM: symmetric.zig:56-70,107-160 with symValue (access.zig:31-42):       */
+/* per element sum = sum + S(i,p) B(p,j) (left) or B(i,p) S(p,j) (right),    */
+/* p ascending, non-fused; then alpha*sum + (beta == 0 ? 0 : beta*C).        */
+/* There is no alpha == 0 quick return in the reference.                      */
+/* ------------------------------------------------------------------------ */
+static inline double sym_val(char uplo, const double *a, int lda, size_t r, size_t c) {
+    int upper = (uplo == 'U' || uplo == 'u');
+    if ((upper && r <= c) || (!upper && r >= c)) return a[mi(lda, r, c)];
+    return a[mi(lda, c, r)];
+}
+
+void oracle_dsymm(char side, char uplo, int m, int n, double alpha, const double *a, int lda,
+                  const double *b, int ldb, double beta, double *c, int ldc) {
+    if (m <= 0 || n <= 0) return;
+    int left = (side == 'L' || side == 'l');
+    for (size_t j = 0; j < (size_t)n; ++j)
+        for (size_t i = 0; i < (size_t)m; ++i) {
+            double sum = 0.0;
+            if (left)
+                for (size_t p = 0; p < (size_t)m; ++p) sum = sum + sym_val(uplo, a, lda, i, p) * b[mi(ldb, p, j)];
+            else
+                for (size_t p = 0; p < (size_t)n; ++p) sum = sum + b[mi(ldb, i, p)] * sym_val(uplo, a, lda, p, j);
+            double *cp = &c[mi(ldc, i, j)];
+            double s = (beta == 0.0) ? 0.0 : beta * *cp;
+            *cp = alpha * sum + s;
+        }
+}
+
+/* ------------------------------------------------------------------------ */
+/* SYR2K: symmetric.zig:301-329,366-412: for the stored triangle,             */
+/* sum = sum + (a_ip b_jp + b_ip a_jp) over p ascending (op(X)(i,p) read by  */
+/* matrixValue), non-fused; then alpha*sum + (beta == 0 ? 0 : beta*C).       */
+/* ------------------------------------------------------------------------ */
+void oracle_dsyr2k(char uplo, char trans, int n, int k, double alpha, const double *a, int lda,
+                   const double *b, int ldb, double beta, double *c, int ldc) {
+    if (n <= 0) return;
+    int upper = (uplo == 'U' || uplo == 'u');
+    int tr = !is_no_trans(trans);
+    size_t kk = (size_t)(k < 0 ? 0 : k);
+    for (size_t j = 0; j < (size_t)n; ++j) {
+        size_t r0 = upper ? 0 : j, r1 = upper ? j + 1 : (size_t)n;
+        for (size_t i = r0; i < r1; ++i) {
+            double sum = 0.0;
+            for (size_t p = 0; p < kk; ++p) {
+                double ai = tr ? a[mi(lda, p, i)] : a[mi(lda, i, p)];
+                double bi = tr ? b[mi(ldb, p, i)] : b[mi(ldb, i, p)];
+                double aj = tr ? a[mi(lda, p, j)] : a[mi(lda, j, p)];
+                double bj = tr ? b[mi(ldb, p, j)] : b[mi(ldb, j, p)];
+                double t1 = ai * bj;
+                double t2 = bi * aj;
+                sum = sum + (t1 + t2);
+            }
+            double *cp = &c[mi(ldc, i, j)];
+            double s = (beta == 0.0) ? 0.0 : beta * *cp;
+            *cp = alpha * sum + s;
+        }
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* TRMM: triangular.zig:202-231.                                              */
+/*  - left: per column, trmv (the scalar trmvUnit of                          */
+/*    matrix_vector/triangular.zig:44-57,59-78,80-133: axpy/dot steps,        */
+/*    non-fused), then B(:,j) = alpha*B(:,j) unless alpha == 1                */
+/*    (scaleDenseColumns, triangular.zig:52-59).  The reference may run       */
+/*    tuned SIMD axpy/dot bodies here; their rounding is parity unpinned.     */
+/*  - right: per row, sum over all p of B(i,p) op(A)(p,j) (triValue gives 0   */
+/*    outside the triangle and 1 on a unit diagonal), B(i,j) = alpha*sum,     */
+/*    columns descending when op(A) is upper so the row is read before it    */
+/*    is overwritten.                                                         */
+/* ------------------------------------------------------------------------ */
+static void trmv_unit(char uplo, int trans, char diag, size_t n, const double *a, int lda, double *x) {
+    int upper = (uplo == 'U' || uplo == 'u');
+    int nonunit = !(diag == 'U' || diag == 'u');
+    if (!trans) {
+        if (upper) {
+            for (size_t j = 0; j < n; ++j) {
+                const double *col = a + j * (size_t)lda;
+                double xj = x[j];
+                if (xj != 0.0) for (size_t i = 0; i < j; ++i) x[i] = x[i] + xj * col[i];
+                if (nonunit) x[j] = xj * col[j];
+            }
+        } else {
+            for (size_t j = n; j-- > 0;) {
+                const double *col = a + j * (size_t)lda;
+                double xj = x[j];
+                if (nonunit) x[j] = xj * col[j];
+                if (xj != 0.0) for (size_t i = j + 1; i < n; ++i) x[i] = x[i] + xj * col[i];
+            }
+        }
+        return;
+    }
+    if (upper) {
+        for (size_t j = n; j-- > 0;) {
+            const double *col = a + j * (size_t)lda;
+            double sum = 0.0;
+            for (size_t i = 0; i < j; ++i) sum = sum + col[i] * x[i];
+            double d = nonunit ? col[j] : 1.0;
+            sum = sum + d * x[j];
+            x[j] = sum;
+        }
+    } else {
+        for (size_t j = 0; j < n; ++j) {
+            const double *col = a + j * (size_t)lda;
+            double d = nonunit ? col[j] : 1.0;
+            double sum = d * x[j];
+            double dot = 0.0;
+            for (size_t i = j + 1; i < n; ++i) dot = dot + col[i] * x[i];
+            sum = sum + dot;
+            x[j] = sum;
+        }
+    }
+}
+
+void oracle_dtrmm(char side, char uplo, char transa, char diag, int m, int n, double alpha,
+                  const double *a, int lda, double *b, int ldb) {
+    if (m <= 0 || n <= 0) return;
+    int trans = !is_no_trans(transa);
+    int upper = (uplo == 'U' || uplo == 'u');
+    if (side == 'L' || side == 'l') {
+        for (size_t j = 0; j < (size_t)n; ++j) {
+            double *x = b + j * (size_t)ldb;
+            trmv_unit(uplo, trans, diag, (size_t)m, a, lda, x);
+            if (alpha != 1.0)
+                for (size_t i = 0; i < (size_t)m; ++i) x[i] = alpha * x[i];
+        }
+        return;
+    }
+    int op_upper = (!trans && upper) || (trans && !upper);
+    for (size_t i = 0; i < (size_t)m; ++i) {
+        for (size_t t = 0; t < (size_t)n; ++t) {
+            size_t jj = op_upper ? (size_t)n - 1 - t : t;
+            double sum = 0.0;
+            for (size_t p = 0; p < (size_t)n; ++p) sum = sum + b[mi(ldb, i, p)] * tri_val(uplo, diag, trans, a, lda, p, jj);
+            b[mi(ldb, i, jj)] = alpha * sum;
+        }
+    }
 }
 
 /* ------------------------------------------------------------------------ */

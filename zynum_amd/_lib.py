@@ -79,6 +79,30 @@ SIGNATURES = {
                            C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_int]),
     "cgemm_": (None, [_CH, _CH, _I, _I, _I, C.c_void_p, C.c_void_p, _I, C.c_void_p, _I, C.c_void_p, C.c_void_p, _I]),
     "zgemm_": (None, [_CH, _CH, _I, _I, _I, C.c_void_p, C.c_void_p, _I, C.c_void_p, _I, C.c_void_p, C.c_void_p, _I]),
+    "cblas_ssymm": (None, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_float, C.c_void_p, C.c_int, C.c_void_p,
+                           C.c_int, C.c_float, C.c_void_p, C.c_int]),
+    "cblas_dsymm": (None, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_void_p, C.c_int, C.c_void_p,
+                           C.c_int, C.c_double, C.c_void_p, C.c_int]),
+    "cblas_ssyr2k": (None, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_float, C.c_void_p, C.c_int, C.c_void_p,
+                            C.c_int, C.c_float, C.c_void_p, C.c_int]),
+    "cblas_dsyr2k": (None, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_void_p, C.c_int, C.c_void_p,
+                            C.c_int, C.c_double, C.c_void_p, C.c_int]),
+    "cblas_strmm": (None, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_float, C.c_void_p,
+                           C.c_int, C.c_void_p, C.c_int]),
+    "cblas_dtrmm": (None, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_void_p,
+                           C.c_int, C.c_void_p, C.c_int]),
+    "ssymm_": (None, [_CH, _CH, _I, _I, _F, C.c_void_p, _I, C.c_void_p, _I, _F, C.c_void_p, _I]),
+    "dsymm_": (None, [_CH, _CH, _I, _I, _D, C.c_void_p, _I, C.c_void_p, _I, _D, C.c_void_p, _I]),
+    "ssyr2k_": (None, [_CH, _CH, _I, _I, _F, C.c_void_p, _I, C.c_void_p, _I, _F, C.c_void_p, _I]),
+    "dsyr2k_": (None, [_CH, _CH, _I, _I, _D, C.c_void_p, _I, C.c_void_p, _I, _D, C.c_void_p, _I]),
+    "strmm_": (None, [_CH, _CH, _CH, _CH, _I, _I, _F, C.c_void_p, _I, C.c_void_p, _I]),
+    "dtrmm_": (None, [_CH, _CH, _CH, _CH, _I, _I, _D, C.c_void_p, _I, C.c_void_p, _I]),
+    "zynum_gpu_dsymm": (C.c_int, [C.c_char, C.c_char, C.c_int, C.c_int, C.c_double, C.c_void_p, C.c_int, C.c_void_p,
+                                  C.c_int, C.c_double, C.c_void_p, C.c_int, C.c_void_p, C.c_int]),
+    "zynum_gpu_dtrmm": (C.c_int, [C.c_char, C.c_char, C.c_char, C.c_char, C.c_int, C.c_int, C.c_double, C.c_void_p,
+                                  C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_int]),
+    "zynum_gpu_dsyr2k": (C.c_int, [C.c_char, C.c_char, C.c_int, C.c_int, C.c_double, C.c_void_p, C.c_int, C.c_void_p,
+                                   C.c_int, C.c_double, C.c_void_p, C.c_int, C.c_void_p, C.c_int]),
     "sgemm_": (None, [_CH, _CH, _I, _I, _I, _F, C.c_void_p, _I, C.c_void_p, _I, _F, C.c_void_p, _I]),
     "dgemm_": (None, [_CH, _CH, _I, _I, _I, _D, C.c_void_p, _I, C.c_void_p, _I, _D, C.c_void_p, _I]),
     "ssyrk_": (None, [_CH, _CH, _I, _I, _F, C.c_void_p, _I, _F, C.c_void_p, _I]),

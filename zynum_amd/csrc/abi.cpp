@@ -1,8 +1,8 @@
 // Exported BLAS symbols and the private GPU-tier request ABI.
 //
 // Argument checking, xerbla reporting and CBLAS row-major normalisation keep
-// Zynum's semantics (src/blas/abi/fortran.zig:16-47,88-117,252-262,277-282,
-// 760-876; src/blas/abi/cblas.zig:16-116,1415-1439,1550-1559,1742-1753).
+// Zynum's semantics (src/blas/abi/fortran.zig:16-47,88-117,239-282,760-876;
+// src/blas/abi/cblas.zig:16-116,1415-1489,1550-1559,1605-1627,1677-1753).
 // Execution is always on the GPU: device operands are used in place and host
 // operands are staged through device memory.  There is no CPU compute path.
 #include <ctype.h>
@@ -65,6 +65,32 @@ int rank_k_error(char uplo, char trans, int32_t n, int32_t k, int32_t lda, int32
     const int32_t nrowa = (trans == 'N') ? n : k;
     if (lda < max1(nrowa)) return 7;
     if (ldc < max1(n)) return 10;
+    return 0;
+}
+
+// fortran.zig:239-250
+int symmetric_error(char side, char uplo, int32_t m, int32_t n, int32_t lda, int32_t ldb, int32_t ldc) {
+    if (!valid_side(side)) return 1;
+    if (!valid_uplo(uplo)) return 2;
+    if (m < 0) return 3;
+    if (n < 0) return 4;
+    const int32_t nrowa = (side == 'L') ? m : n;
+    if (lda < max1(nrowa)) return 7;
+    if (ldb < max1(m)) return 9;
+    if (ldc < max1(m)) return 12;
+    return 0;
+}
+
+// fortran.zig:264-275
+int rank_2k_error(char uplo, char trans, int32_t n, int32_t k, int32_t lda, int32_t ldb, int32_t ldc) {
+    if (!valid_uplo(uplo)) return 1;
+    if (!valid_trans(trans)) return 2;
+    if (n < 0) return 3;
+    if (k < 0) return 4;
+    const int32_t nrow = (trans == 'N') ? n : k;
+    if (lda < max1(nrow)) return 7;
+    if (ldb < max1(nrow)) return 9;
+    if (ldc < max1(n)) return 12;
     return 0;
 }
 
@@ -211,6 +237,72 @@ int syrk_op(char uplo, bool trans, int n, int k, T alpha, const T* a, int64_t ld
     return ZYNUM_GPU_OK;
 }
 
+template <typename T>
+int symm_op(char side, char uplo, int m, int n, T alpha, const T* a, int64_t lda, const T* b, int64_t ldb, T beta,
+            T* c, int64_t ldc, hipStream_t s, int flags) {
+    if (m <= 0 || n <= 0) return ZYNUM_GPU_OK;  // symmetric.zig:108
+    if (take_forced_decline()) return ZYNUM_GPU_DECLINE_FORCED;
+    if (!device_ready()) return ZYNUM_GPU_DECLINE_NO_DEVICE;
+    const int na = (side == 'L') ? m : n;
+    Operand oa, ob, oc;
+    int st = ZYNUM_GPU_OK;
+    if (alpha != T(0)) {
+        st = oa.bind_in(a, na, na, lda, sizeof(T), s);
+        if (st == ZYNUM_GPU_OK) st = ob.bind_in(b, m, n, ldb, sizeof(T), s);
+    }
+    if (st == ZYNUM_GPU_OK) st = oc.bind_out(c, m, n, ldc, sizeof(T), beta != T(0), s);
+    if (st != ZYNUM_GPU_OK) return st;
+    st = symm_device<T>(side, uplo, m, n, alpha, static_cast<const T*>(oa.dev), lda, static_cast<const T*>(ob.dev), ldb,
+                        beta, static_cast<T*>(oc.dev), ldc, s);
+    if (st != ZYNUM_GPU_OK) return st;
+    if (oc.staged) return oc.finish(s);
+    if (!(flags & ZYNUM_GPU_FLAG_ASYNC) || oa.staged || ob.staged) return status_from_hip(hipStreamSynchronize(s));
+    return ZYNUM_GPU_OK;
+}
+
+template <typename T>
+int trmm_op(char side, char uplo, bool trans, char diag, int m, int n, T alpha, const T* a, int64_t lda, T* b,
+            int64_t ldb, hipStream_t s, int flags) {
+    if (m <= 0 || n <= 0) return ZYNUM_GPU_OK;  // triangular.zig:203
+    if (take_forced_decline()) return ZYNUM_GPU_DECLINE_FORCED;
+    if (!device_ready()) return ZYNUM_GPU_DECLINE_NO_DEVICE;
+    const int na = (side == 'L') ? m : n;
+    Operand oa, ob;
+    int st = ZYNUM_GPU_OK;
+    if (alpha != T(0)) st = oa.bind_in(a, na, na, lda, sizeof(T), s);
+    if (st == ZYNUM_GPU_OK) st = ob.bind_out(b, m, n, ldb, sizeof(T), alpha != T(0), s);
+    if (st != ZYNUM_GPU_OK) return st;
+    st = trmm_device<T>(side, uplo, trans, diag, m, n, alpha, static_cast<const T*>(oa.dev), lda,
+                        static_cast<T*>(ob.dev), ldb, s);
+    if (st != ZYNUM_GPU_OK) return st;
+    if (ob.staged) return ob.finish(s);
+    if (!(flags & ZYNUM_GPU_FLAG_ASYNC) || oa.staged) return status_from_hip(hipStreamSynchronize(s));
+    return ZYNUM_GPU_OK;
+}
+
+template <typename T>
+int syr2k_op(char uplo, bool trans, int n, int k, T alpha, const T* a, int64_t lda, const T* b, int64_t ldb, T beta,
+             T* c, int64_t ldc, hipStream_t s, int flags) {
+    if (n <= 0) return ZYNUM_GPU_OK;  // symmetric.zig:367
+    if (take_forced_decline()) return ZYNUM_GPU_DECLINE_FORCED;
+    if (!device_ready()) return ZYNUM_GPU_DECLINE_NO_DEVICE;
+    Operand oa, ob, oc;
+    int st = ZYNUM_GPU_OK;
+    if (k > 0 && alpha != T(0)) {
+        st = oa.bind_in(a, trans ? k : n, trans ? n : k, lda, sizeof(T), s);
+        if (st == ZYNUM_GPU_OK) st = ob.bind_in(b, trans ? k : n, trans ? n : k, ldb, sizeof(T), s);
+    }
+    // the unstored triangle must come back unchanged, so C is always copied in
+    if (st == ZYNUM_GPU_OK) st = oc.bind_out(c, n, n, ldc, sizeof(T), true, s);
+    if (st != ZYNUM_GPU_OK) return st;
+    st = syr2k_device<T>(uplo, trans, n, k, alpha, static_cast<const T*>(oa.dev), lda, static_cast<const T*>(ob.dev),
+                         ldb, beta, static_cast<T*>(oc.dev), ldc, s);
+    if (st != ZYNUM_GPU_OK) return st;
+    if (oc.staged) return oc.finish(s);
+    if (!(flags & ZYNUM_GPU_FLAG_ASYNC) || oa.staged || ob.staged) return status_from_hip(hipStreamSynchronize(s));
+    return ZYNUM_GPU_OK;
+}
+
 template int gemm_op<double>(bool, bool, int, int, int, double, const double*, int64_t, const double*, int64_t,
                              double, double*, int64_t, hipStream_t, int, int, int, int);
 template int gemm_op<float>(bool, bool, int, int, int, float, const float*, int64_t, const float*, int64_t, float,
@@ -306,6 +398,61 @@ void dsyrk_(const char* u, const char* t, const zynum_blas_int* n, const zynum_b
                                              zg::blas_stream(), ZYNUM_GPU_MODE_PLAIN, 0, 0));
 }
 
+// fortran.zig:777-784
+void ssymm_(const char* side, const char* u, const zynum_blas_int* m, const zynum_blas_int* n, const float* alpha,
+            const float* a, const zynum_blas_int* lda, const float* b, const zynum_blas_int* ldb, const float* beta,
+            float* c, const zynum_blas_int* ldc) {
+    const char S = up(side), U = up(u);
+    if (report_error("SSYMM ", symmetric_error(S, U, *m, *n, *lda, *ldb, *ldc))) return;
+    note_status("SSYMM", zg::symm_op<float>(S, U, *m, *n, *alpha, a, *lda, b, *ldb, *beta, c, *ldc, zg::blas_stream(), 0));
+}
+
+void dsymm_(const char* side, const char* u, const zynum_blas_int* m, const zynum_blas_int* n, const double* alpha,
+            const double* a, const zynum_blas_int* lda, const double* b, const zynum_blas_int* ldb, const double* beta,
+            double* c, const zynum_blas_int* ldc) {
+    const char S = up(side), U = up(u);
+    if (report_error("DSYMM ", symmetric_error(S, U, *m, *n, *lda, *ldb, *ldc))) return;
+    note_status("DSYMM",
+                zg::symm_op<double>(S, U, *m, *n, *alpha, a, *lda, b, *ldb, *beta, c, *ldc, zg::blas_stream(), 0));
+}
+
+// fortran.zig:827-834
+void ssyr2k_(const char* u, const char* t, const zynum_blas_int* n, const zynum_blas_int* k, const float* alpha,
+             const float* a, const zynum_blas_int* lda, const float* b, const zynum_blas_int* ldb, const float* beta,
+             float* c, const zynum_blas_int* ldc) {
+    const char U = up(u), T = up(t);
+    if (report_error("SSYR2K", rank_2k_error(U, T, *n, *k, *lda, *ldb, *ldc))) return;
+    note_status("SSYR2K", zg::syr2k_op<float>(U, T != 'N', *n, *k, *alpha, a, *lda, b, *ldb, *beta, c, *ldc,
+                                              zg::blas_stream(), 0));
+}
+
+void dsyr2k_(const char* u, const char* t, const zynum_blas_int* n, const zynum_blas_int* k, const double* alpha,
+             const double* a, const zynum_blas_int* lda, const double* b, const zynum_blas_int* ldb,
+             const double* beta, double* c, const zynum_blas_int* ldc) {
+    const char U = up(u), T = up(t);
+    if (report_error("DSYR2K", rank_2k_error(U, T, *n, *k, *lda, *ldb, *ldc))) return;
+    note_status("DSYR2K", zg::syr2k_op<double>(U, T != 'N', *n, *k, *alpha, a, *lda, b, *ldb, *beta, c, *ldc,
+                                               zg::blas_stream(), 0));
+}
+
+// fortran.zig:852-859
+void strmm_(const char* side, const char* u, const char* t, const char* d, const zynum_blas_int* m,
+            const zynum_blas_int* n, const float* alpha, const float* a, const zynum_blas_int* lda, float* b,
+            const zynum_blas_int* ldb) {
+    const char S = up(side), U = up(u), T = up(t), D = up(d);
+    if (report_error("STRMM ", triangular_error(S, U, T, D, *m, *n, *lda, *ldb))) return;
+    note_status("STRMM", zg::trmm_op<float>(S, U, T != 'N', D, *m, *n, *alpha, a, *lda, b, *ldb, zg::blas_stream(), 0));
+}
+
+void dtrmm_(const char* side, const char* u, const char* t, const char* d, const zynum_blas_int* m,
+            const zynum_blas_int* n, const double* alpha, const double* a, const zynum_blas_int* lda, double* b,
+            const zynum_blas_int* ldb) {
+    const char S = up(side), U = up(u), T = up(t), D = up(d);
+    if (report_error("DTRMM ", triangular_error(S, U, T, D, *m, *n, *lda, *ldb))) return;
+    note_status("DTRMM",
+                zg::trmm_op<double>(S, U, T != 'N', D, *m, *n, *alpha, a, *lda, b, *ldb, zg::blas_stream(), 0));
+}
+
 void strsm_(const char* side, const char* u, const char* t, const char* d, const zynum_blas_int* m,
             const zynum_blas_int* n, const float* alpha, const float* a, const zynum_blas_int* lda, float* b,
             const zynum_blas_int* ldb) {
@@ -382,6 +529,78 @@ void cblas_zgemm(int layout, int transa, int transb, int m, int n, int k, const 
     typedef zynum_blas_complex_double cd;
     zgemm_(&ta, &tb, &mm, &nn, &kk, static_cast<const cd*>(alpha), static_cast<const cd*>(row ? b : a), &la,
            static_cast<const cd*>(row ? a : b), &lb, static_cast<const cd*>(beta), static_cast<cd*>(c), &lc);
+}
+
+// cblas.zig:1467-1489: only the layout is checked here; side and uplo map to
+// 'X' when invalid and the Fortran check reports them.
+void cblas_ssymm(int layout, int side, int uplo, int m, int n, float alpha, const float* a, int lda, const float* b,
+                 int ldb, float beta, float* c, int ldc) {
+    if (!cb_layout(layout)) return;
+    const bool row = layout == CblasRowMajor;
+    char ss = row ? cb_row_side_char(side) : cb_side_char(side);
+    char uu = row ? cb_row_uplo_char(uplo) : cb_uplo_char(uplo);
+    zynum_blas_int mm = row ? n : m, nn = row ? m : n, la = lda, lb = ldb, lc = ldc;
+    ssymm_(&ss, &uu, &mm, &nn, &alpha, a, &la, b, &lb, &beta, c, &lc);
+}
+
+void cblas_dsymm(int layout, int side, int uplo, int m, int n, double alpha, const double* a, int lda,
+                 const double* b, int ldb, double beta, double* c, int ldc) {
+    if (!cb_layout(layout)) return;
+    const bool row = layout == CblasRowMajor;
+    char ss = row ? cb_row_side_char(side) : cb_side_char(side);
+    char uu = row ? cb_row_uplo_char(uplo) : cb_uplo_char(uplo);
+    zynum_blas_int mm = row ? n : m, nn = row ? m : n, la = lda, lb = ldb, lc = ldc;
+    dsymm_(&ss, &uu, &mm, &nn, &alpha, a, &la, b, &lb, &beta, c, &lc);
+}
+
+// cblas.zig:1605-1627
+void cblas_ssyr2k(int layout, int uplo, int trans, int n, int k, float alpha, const float* a, int lda, const float* b,
+                  int ldb, float beta, float* c, int ldc) {
+    if (!cb_layout(layout)) return;
+    const bool row = layout == CblasRowMajor;
+    char uu = row ? cb_row_uplo_char(uplo) : cb_uplo_char(uplo);
+    char tt = row ? cb_row_trans_char(trans) : cb_trans_char(trans);
+    zynum_blas_int nn = n, kk = k, la = lda, lb = ldb, lc = ldc;
+    ssyr2k_(&uu, &tt, &nn, &kk, &alpha, a, &la, b, &lb, &beta, c, &lc);
+}
+
+void cblas_dsyr2k(int layout, int uplo, int trans, int n, int k, double alpha, const double* a, int lda,
+                  const double* b, int ldb, double beta, double* c, int ldc) {
+    if (!cb_layout(layout)) return;
+    const bool row = layout == CblasRowMajor;
+    char uu = row ? cb_row_uplo_char(uplo) : cb_uplo_char(uplo);
+    char tt = row ? cb_row_trans_char(trans) : cb_trans_char(trans);
+    zynum_blas_int nn = n, kk = k, la = lda, lb = ldb, lc = ldc;
+    dsyr2k_(&uu, &tt, &nn, &kk, &alpha, a, &la, b, &lb, &beta, c, &lc);
+}
+
+// cblas.zig:1677-1700: every enum is checked (invalid returns silently)
+void cblas_strmm(int layout, int side, int uplo, int transa, int diag, int m, int n, float alpha, const float* a,
+                 int lda, float* b, int ldb) {
+    if (!cb_layout(layout) || !(side == CblasLeft || side == CblasRight) ||
+        !(uplo == CblasUpper || uplo == CblasLower) || !cb_trans(transa) ||
+        !(diag == CblasNonUnit || diag == CblasUnit))
+        return;
+    const bool row = layout == CblasRowMajor;
+    char ss = row ? cb_row_side_char(side) : cb_side_char(side);
+    char uu = row ? cb_row_uplo_char(uplo) : cb_uplo_char(uplo);
+    char tt = cb_trans_char(transa), dd = cb_diag_char(diag);
+    zynum_blas_int mm = row ? n : m, nn = row ? m : n, la = lda, lb = ldb;
+    strmm_(&ss, &uu, &tt, &dd, &mm, &nn, &alpha, a, &la, b, &lb);
+}
+
+void cblas_dtrmm(int layout, int side, int uplo, int transa, int diag, int m, int n, double alpha, const double* a,
+                 int lda, double* b, int ldb) {
+    if (!cb_layout(layout) || !(side == CblasLeft || side == CblasRight) ||
+        !(uplo == CblasUpper || uplo == CblasLower) || !cb_trans(transa) ||
+        !(diag == CblasNonUnit || diag == CblasUnit))
+        return;
+    const bool row = layout == CblasRowMajor;
+    char ss = row ? cb_row_side_char(side) : cb_side_char(side);
+    char uu = row ? cb_row_uplo_char(uplo) : cb_uplo_char(uplo);
+    char tt = cb_trans_char(transa), dd = cb_diag_char(diag);
+    zynum_blas_int mm = row ? n : m, nn = row ? m : n, la = lda, lb = ldb;
+    dtrmm_(&ss, &uu, &tt, &dd, &mm, &nn, &alpha, a, &la, b, &lb);
 }
 
 void cblas_dsyrk(int layout, int uplo, int trans, int n, int k, double alpha, const double* a, int lda, double beta,
@@ -469,6 +688,30 @@ int zynum_gpu_dsyrk(char uplo, char trans, int n, int k, double alpha, const dou
     if (rank_k_error(U, T, n, k, lda, ldc) != 0) return ZYNUM_GPU_DECLINE_ARGUMENT;
     hipStream_t s = stream ? reinterpret_cast<hipStream_t>(stream) : zg::blas_stream();
     return zg::syrk_op<double>(U, T != 'N', n, k, alpha, a, lda, beta, c, ldc, s, mode, mode_param, flags);
+}
+
+int zynum_gpu_dsymm(char side, char uplo, int m, int n, double alpha, const double* a, int lda, const double* b,
+                    int ldb, double beta, double* c, int ldc, void* stream, int flags) {
+    const char S = (char)toupper(side), U = (char)toupper(uplo);
+    if (symmetric_error(S, U, m, n, lda, ldb, ldc) != 0) return ZYNUM_GPU_DECLINE_ARGUMENT;
+    hipStream_t s = stream ? reinterpret_cast<hipStream_t>(stream) : zg::blas_stream();
+    return zg::symm_op<double>(S, U, m, n, alpha, a, lda, b, ldb, beta, c, ldc, s, flags);
+}
+
+int zynum_gpu_dtrmm(char side, char uplo, char transa, char diag, int m, int n, double alpha, const double* a,
+                    int lda, double* b, int ldb, void* stream, int flags) {
+    const char S = (char)toupper(side), U = (char)toupper(uplo), T = (char)toupper(transa), D = (char)toupper(diag);
+    if (triangular_error(S, U, T, D, m, n, lda, ldb) != 0) return ZYNUM_GPU_DECLINE_ARGUMENT;
+    hipStream_t s = stream ? reinterpret_cast<hipStream_t>(stream) : zg::blas_stream();
+    return zg::trmm_op<double>(S, U, T != 'N', D, m, n, alpha, a, lda, b, ldb, s, flags);
+}
+
+int zynum_gpu_dsyr2k(char uplo, char trans, int n, int k, double alpha, const double* a, int lda, const double* b,
+                     int ldb, double beta, double* c, int ldc, void* stream, int flags) {
+    const char U = (char)toupper(uplo), T = (char)toupper(trans);
+    if (rank_2k_error(U, T, n, k, lda, ldb, ldc) != 0) return ZYNUM_GPU_DECLINE_ARGUMENT;
+    hipStream_t s = stream ? reinterpret_cast<hipStream_t>(stream) : zg::blas_stream();
+    return zg::syr2k_op<double>(U, T != 'N', n, k, alpha, a, lda, b, ldb, beta, c, ldc, s, flags);
 }
 
 int zynum_gpu_last_status(void) { return t_last_status; }

@@ -340,6 +340,209 @@ int syrk_device(char uplo, bool trans, int n, int k, T alpha, const T* a, int64_
     return status_from_hip(hipGetLastError());
 }
 
+// ---------------------------------------------------------------------------
+// SYMM, TRMM, SYR2K on the GEMM core
+// ---------------------------------------------------------------------------
+inline int64_t ld_round(int64_t rows) { return (rows + 7) / 8 * 8; }
+inline size_t bytes_round(size_t b) { return (b + 255) / 256 * 256; }
+
+// S (n x n) <- the full symmetric matrix whose stored triangle is A's
+// (symValue, access.zig:31-42).  One 32 x 32 tile per workgroup: the mirrored
+// tile of A is read along its contiguous dimension and transposed in LDS.
+template <typename T>
+__global__ __launch_bounds__(256) void symm_fill_kernel(int n, int lower, const T* __restrict__ a, int64_t lda,
+                                                        T* __restrict__ s, int64_t lds, int tiles) {
+    __shared__ T t[32][33];
+    const int bi = (int)(blockIdx.x % (unsigned)tiles), bj = (int)(blockIdx.x / (unsigned)tiles);
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+#pragma unroll
+    for (int c = ty; c < 32; c += 8) {  // t[c][r] = A(bj*32 + r, bi*32 + c)
+        const int r = bj * 32 + tx, cc = bi * 32 + c;
+        if (r < n && cc < n) t[c][tx] = a[(int64_t)cc * lda + r];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int c = ty; c < 32; c += 8) {  // S(i, j), i = bi*32 + tx, j = bj*32 + c
+        const int i = bi * 32 + tx, j = bj * 32 + c;
+        if (i < n && j < n) {
+            const bool stored = lower ? (i >= j) : (i <= j);
+            s[(int64_t)j * lds + i] = stored ? a[(int64_t)j * lda + i] : t[tx][c];
+        }
+    }
+}
+
+// T (n x n) <- the triangle of A, zeros elsewhere, ones on a unit diagonal
+// (triValue, access.zig:54-63, without the transpose).
+template <typename T>
+__global__ void trmm_fill_kernel(int n, int lower, int unit, const T* __restrict__ a, int64_t lda, T* __restrict__ t,
+                                 int64_t ldt) {
+    const int64_t total = (int64_t)n * n;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t j = e / n, i = e % n;
+        T v = T(0);
+        if (i == j) v = unit ? T(1) : a[j * lda + i];
+        else if (lower ? (i > j) : (i < j)) v = a[j * lda + i];
+        t[j * ldt + i] = v;
+    }
+}
+
+// C = alpha * S * B + beta * C (side L, S m x m) or alpha * B * S + beta * C
+// (side R, S n x n), S symmetric from the stored triangle of A.  The reference
+// (symmetric.zig:56-70) forms each element as one sum over the full row of S;
+// here S is expanded once into the workspace and the tuned GEMM runs on it.
+template <typename T>
+int symm_device(char side, char uplo, int m, int n, T alpha, const T* a, int64_t lda, const T* b, int64_t ldb, T beta,
+                T* c, int64_t ldc, hipStream_t s) {
+    if (m <= 0 || n <= 0) return ZYNUM_GPU_OK;
+    const bool left = (side == 'L');
+    GemmDesc<T> d;
+    d.m = m; d.n = n;
+    d.alpha = alpha; d.beta = beta;
+    d.c = c; d.ldc = ldc;
+    if (alpha == T(0)) {  // C = beta * C (exact 0 for beta == 0)
+        d.k = 0;
+        return status_from_hip(gemm_launch<T>(d, s));
+    }
+    const int na = left ? m : n;
+    const int64_t lds = ld_round(na);
+    T* sym = static_cast<T*>(workspace_acquire(bytes_round((size_t)lds * na * sizeof(T))));
+    if (!sym) return ZYNUM_GPU_DECLINE_ALLOC;
+    const int tiles = (na + 31) / 32;
+    const int64_t blocks = (int64_t)tiles * tiles;
+    hipError_t e = hipSuccess;
+    if (blocks > 0x7fffffff) e = hipErrorInvalidValue;
+    if (e == hipSuccess) {
+        symm_fill_kernel<T><<<(unsigned)blocks, 256, 0, s>>>(na, uplo == 'L', a, lda, sym, lds, tiles);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) {
+        d.ta = false; d.tb = false;
+        if (left) {
+            d.k = m; d.a = sym; d.lda = lds; d.b = b; d.ldb = ldb;
+        } else {
+            d.k = n; d.a = b; d.lda = ldb; d.b = sym; d.ldb = lds;
+        }
+        e = gemm_launch<T>(d, s);
+    }
+    int st = status_from_hip(e);
+    if (st == ZYNUM_GPU_OK) st = status_from_hip(hipStreamSynchronize(s));  // workspace reuse is host-ordered
+    workspace_release();
+    return st;
+}
+
+// B = alpha * op(A) * B (side L) or alpha * B * op(A) (side R), A triangular.
+// The triangle is expanded into a zero-filled square matrix and B copied
+// aside, then the product runs as GEMMs over row (L) or column (R) panels of
+// the result, each with only the K range where op(A) is not zero, so the work
+// is about half a full GEMM.  triangular.zig:202-231 multiplies column by
+// column (trmv) or row by row.
+template <typename T>
+int trmm_device(char side, char uplo, bool trans, char diag, int m, int n, T alpha, const T* a, int64_t lda, T* b,
+                int64_t ldb, hipStream_t s) {
+    if (m <= 0 || n <= 0) return ZYNUM_GPU_OK;
+    if (alpha == T(0)) return status_from_hip(scale_launch<T>(m, n, T(0), b, ldb, s));
+    const bool left = (side == 'L');
+    const int na = left ? m : n;
+    const int64_t ldt = ld_round(na), ldw = ld_round(m);
+    const size_t t_bytes = bytes_round((size_t)ldt * na * sizeof(T));
+    char* ws = static_cast<char*>(workspace_acquire(t_bytes + (size_t)ldw * n * sizeof(T)));
+    if (!ws) return ZYNUM_GPU_DECLINE_ALLOC;
+    T* tri = reinterpret_cast<T*>(ws);
+    T* b0 = reinterpret_cast<T*>(ws + t_bytes);
+    const int64_t total = (int64_t)na * na;
+    const int blocks = (int)std::min<int64_t>(65536, (total + 255) / 256);
+    trmm_fill_kernel<T><<<blocks, 256, 0, s>>>(na, uplo == 'L', diag == 'U', a, lda, tri, ldt);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess)
+        e = hipMemcpy2DAsync(b0, (size_t)ldw * sizeof(T), b, (size_t)ldb * sizeof(T), (size_t)m * sizeof(T), (size_t)n,
+                             hipMemcpyDeviceToDevice, s);
+    const bool lower_eff = (uplo == 'L') != trans;  // op(A) lower triangular
+    const int extent = left ? m : n;               // panels along this dimension
+    int panel = (extent + 7) / 8;
+    panel = std::max(256, (panel + 127) / 128 * 128);
+    for (int x0 = 0; e == hipSuccess && x0 < extent; x0 += panel) {
+        const int x1 = std::min(extent, x0 + panel);
+        GemmDesc<T> d;
+        d.alpha = alpha;
+        d.beta = T(0);
+        if (left) {  // rows x0..x1 of op(A) are nonzero in columns [0, x1) (lower) or [x0, m) (upper)
+            const int k0 = lower_eff ? 0 : x0, k1 = lower_eff ? x1 : m;
+            d.m = x1 - x0; d.n = n; d.k = k1 - k0;
+            d.ta = trans;
+            d.a = trans ? tri + k0 + (int64_t)x0 * ldt : tri + x0 + (int64_t)k0 * ldt;
+            d.lda = ldt;
+            d.tb = false; d.b = b0 + k0; d.ldb = ldw;
+            d.c = b + x0; d.ldc = ldb;
+        } else {  // columns x0..x1 of op(A) are nonzero in rows [x0, n) (lower) or [0, x1) (upper)
+            const int k0 = lower_eff ? x0 : 0, k1 = lower_eff ? n : x1;
+            d.m = m; d.n = x1 - x0; d.k = k1 - k0;
+            d.ta = false; d.a = b0 + (int64_t)k0 * ldw; d.lda = ldw;
+            d.tb = trans;
+            d.b = trans ? tri + x0 + (int64_t)k0 * ldt : tri + k0 + (int64_t)x0 * ldt;
+            d.ldb = ldt;
+            d.c = b + (int64_t)x0 * ldb; d.ldc = ldb;
+        }
+        e = gemm_launch<T>(d, s);
+    }
+    int st = status_from_hip(e);
+    if (st == ZYNUM_GPU_OK) st = status_from_hip(hipStreamSynchronize(s));
+    workspace_release();
+    return st;
+}
+
+// C_tri = alpha * (op(A) op(B)^T + op(B) op(A)^T) + beta * C_tri on the stored
+// triangle (symmetric.zig:301-329): two launches of the triangle-tile kernel,
+// the second adding alpha * op(B) op(A)^T to the first's result.
+template <typename T>
+int syr2k_device(char uplo, bool trans, int n, int k, T alpha, const T* a, int64_t lda, const T* b, int64_t ldb,
+                 T beta, T* c, int64_t ldc, hipStream_t s) {
+    if (n <= 0) return ZYNUM_GPU_OK;
+    const bool lower = (uplo == 'L');
+    if (k <= 0 || alpha == T(0)) {
+        dim3 grid((n + 255) / 256, n);
+        syrk_commit_kernel<T><<<grid, 256, 0, s>>>(n, lower, nullptr, 0, beta, c, ldc);
+        return status_from_hip(hipGetLastError());
+    }
+    GemmArgs<T> p;
+    p.m = n; p.n = n; p.k = k;
+    p.ldc = ldc;
+    p.alpha = alpha;
+    p.c = c;
+    const int tn = (n + SyrkCfg::BM - 1) / SyrkCfg::BM;
+    p.tiles_m = p.tiles_n = tn;
+    p.group_m = 1;
+    const int tiles = tn * (tn + 1) / 2;
+    hipError_t e = hipSuccess;
+    for (int pass = 0; pass < 2 && e == hipSuccess; ++pass) {
+        p.a = pass ? b : a;
+        p.lda = pass ? ldb : lda;
+        p.b = pass ? a : b;
+        p.ldb = pass ? lda : ldb;
+        p.beta = pass ? T(1) : beta;
+        if (lower) {
+            if (trans) syrk_kernel<T, true, 1><<<tiles, SyrkCfg::NT, 0, s>>>(p, tiles);
+            else syrk_kernel<T, false, 1><<<tiles, SyrkCfg::NT, 0, s>>>(p, tiles);
+        } else {
+            if (trans) syrk_kernel<T, true, 2><<<tiles, SyrkCfg::NT, 0, s>>>(p, tiles);
+            else syrk_kernel<T, false, 2><<<tiles, SyrkCfg::NT, 0, s>>>(p, tiles);
+        }
+        e = hipGetLastError();
+    }
+    return status_from_hip(e);
+}
+
+template int symm_device<double>(char, char, int, int, double, const double*, int64_t, const double*, int64_t, double,
+                                 double*, int64_t, hipStream_t);
+template int symm_device<float>(char, char, int, int, float, const float*, int64_t, const float*, int64_t, float,
+                                float*, int64_t, hipStream_t);
+template int trmm_device<double>(char, char, bool, char, int, int, double, const double*, int64_t, double*, int64_t,
+                                 hipStream_t);
+template int trmm_device<float>(char, char, bool, char, int, int, float, const float*, int64_t, float*, int64_t,
+                                hipStream_t);
+template int syr2k_device<double>(char, bool, int, int, double, const double*, int64_t, const double*, int64_t,
+                                  double, double*, int64_t, hipStream_t);
+template int syr2k_device<float>(char, bool, int, int, float, const float*, int64_t, const float*, int64_t, float,
+                                 float*, int64_t, hipStream_t);
 template int trsm_device<double>(char, char, bool, char, int, int, double, const double*, int64_t, double*, int64_t,
                                  hipStream_t, int, int);
 template int trsm_device<float>(char, char, bool, char, int, int, float, const float*, int64_t, float*, int64_t,

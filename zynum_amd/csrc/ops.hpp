@@ -20,6 +20,18 @@ template <typename T>
 int syrk_op(char uplo, bool trans, int n, int k, T alpha, const T* a, int64_t lda, T beta, T* c, int64_t ldc,
             hipStream_t s, int mode, int mode_param, int flags);
 
+template <typename T>
+int symm_op(char side, char uplo, int m, int n, T alpha, const T* a, int64_t lda, const T* b, int64_t ldb, T beta,
+            T* c, int64_t ldc, hipStream_t s, int flags);
+
+template <typename T>
+int trmm_op(char side, char uplo, bool trans, char diag, int m, int n, T alpha, const T* a, int64_t lda, T* b,
+            int64_t ldb, hipStream_t s, int flags);
+
+template <typename T>
+int syr2k_op(char uplo, bool trans, int n, int k, T alpha, const T* a, int64_t lda, const T* b, int64_t ldb, T beta,
+             T* c, int64_t ldc, hipStream_t s, int flags);
+
 // Complex GEMM; a, b, c point at interleaved (re, im) storage, leading
 // dimensions and alpha/beta pointers are in complex elements.  ta/tb: 'N', 'T', 'C'.
 template <typename T>
@@ -41,5 +53,14 @@ int trsm_device(char side, char uplo, bool trans, char diag, int m, int n, T alp
 template <typename T>
 int syrk_device(char uplo, bool trans, int n, int k, T alpha, const T* a, int64_t lda, T beta, T* c, int64_t ldc,
                 hipStream_t s, int mode, int mode_param);
+template <typename T>
+int symm_device(char side, char uplo, int m, int n, T alpha, const T* a, int64_t lda, const T* b, int64_t ldb, T beta,
+                T* c, int64_t ldc, hipStream_t s);
+template <typename T>
+int trmm_device(char side, char uplo, bool trans, char diag, int m, int n, T alpha, const T* a, int64_t lda, T* b,
+                int64_t ldb, hipStream_t s);
+template <typename T>
+int syr2k_device(char uplo, bool trans, int n, int k, T alpha, const T* a, int64_t lda, const T* b, int64_t ldb,
+                 T beta, T* c, int64_t ldc, hipStream_t s);
 
 }  // namespace zg
