@@ -34,10 +34,10 @@ PLAIN_TILES = 11         # kTileConfigs
 GROUPED_TILES = 3        # kGroupedCfgs
 I8_TILES = 14            # kI8Tiles
 PLAIN_SIZES = (8192, 2048, 1024)
-PLAIN_CLASS_MIN_TILES = {8192: 512, 2048: 64, 1024: 0}   # 128x128 tiles: 4096, 256, 64 at these sizes
+PLAIN_CLASS_MIN_TILES = {8192: 512, 2048: 128, 1024: 0}  # 128x128 tiles: 4096, 256, 64 at these sizes
 GROUPED_BUCKETS = ((16, 48), (48, 96), (96, 192), (192, 512))
 I8_SIZES = (8192, 2048)
-I8_CLASS_MIN_TILES = {8192: 64, 2048: 0}                # 256x256 tiles: 1024, 64
+I8_CLASS_MIN_TILES = {8192: 128, 2048: 0}               # 256x256 tiles: 1024, 64
 
 
 def _time(torch, stream, fn, reps: int) -> float:

@@ -6,6 +6,7 @@
 #include <atomic>
 #include <mutex>
 #include <string.h>
+#include <vector>
 
 namespace zg {
 
@@ -18,6 +19,93 @@ std::atomic<int> g_forced{0};
 std::mutex g_ws_mutex;
 void* g_ws_ptr = nullptr;
 size_t g_ws_bytes = 0;
+
+// Staging buffers for host operands: device buffers the kernels read and
+// pinned host bounce buffers the DMA engine copies to and from in one
+// transfer.  A host-pointer call would otherwise pay a hipMalloc / hipFree
+// pair (and the device-wide wait of hipFree) per operand, and a 2D copy from
+// pageable memory moves one column per transfer.  Idle buffers are kept up to
+// a byte cap and handed to any operand that fits in at most twice its size.
+class BufCache {
+  public:
+    BufCache(bool pinned, size_t cap) : pinned_(pinned), cap_(cap) {}
+
+    void* acquire(size_t bytes, size_t* got) {
+        {
+            std::lock_guard<std::mutex> lock(mu_);
+            size_t best = free_.size();
+            for (size_t i = 0; i < free_.size(); ++i) {
+                const size_t b = free_[i].bytes;
+                if (b >= bytes && b <= 2 * bytes + kGranule && (best == free_.size() || b < free_[best].bytes))
+                    best = i;
+            }
+            if (best < free_.size()) {
+                Buf buf = free_[best];
+                free_.erase(free_.begin() + (ptrdiff_t)best);
+                cached_ -= buf.bytes;
+                *got = buf.bytes;
+                return buf.ptr;
+            }
+        }
+        const size_t want = (bytes + kGranule - 1) / kGranule * kGranule;
+        void* p = nullptr;
+        const hipError_t e = pinned_ ? hipHostMalloc(&p, want, hipHostMallocDefault) : hipMalloc(&p, want);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            return nullptr;
+        }
+        *got = want;
+        return p;
+    }
+
+    // The caller has synchronised the stream that used the buffer.
+    void release(void* p, size_t bytes) {
+        {
+            std::lock_guard<std::mutex> lock(mu_);
+            if (cached_ + bytes <= cap_) {
+                free_.push_back({p, bytes});
+                cached_ += bytes;
+                return;
+            }
+        }
+        drop(p);
+    }
+
+    void free_all() {
+        std::lock_guard<std::mutex> lock(mu_);
+        for (const Buf& b : free_) drop(b.ptr);
+        free_.clear();
+        cached_ = 0;
+    }
+
+  private:
+    struct Buf {
+        void* ptr;
+        size_t bytes;
+    };
+    static constexpr size_t kGranule = size_t(2) << 20;
+    void drop(void* p) { (void)(pinned_ ? hipHostFree(p) : hipFree(p)); }
+    bool pinned_;
+    size_t cap_;
+    std::mutex mu_;
+    std::vector<Buf> free_;
+    size_t cached_ = 0;
+};
+
+BufCache g_stage_dev(false, size_t(8) << 30);
+BufCache g_stage_host(true, size_t(2) << 30);
+
+// Copy the logical rows x cols part of a column-major matrix between two
+// buffers with the same leading dimension (one memcpy when there is no padding).
+void copy_columns(void* dst, const void* src, int64_t rows, int64_t cols, int64_t ld, size_t es) {
+    if (rows == ld) {
+        memcpy(dst, src, (size_t)(rows * cols) * es);
+        return;
+    }
+    for (int64_t j = 0; j < cols; ++j)
+        memcpy(static_cast<char*>(dst) + (size_t)(j * ld) * es, static_cast<const char*>(src) + (size_t)(j * ld) * es,
+               (size_t)rows * es);
+}
 }  // namespace
 
 bool device_ready() {
@@ -71,22 +159,28 @@ int status_from_hip(hipError_t e) {
     return ZYNUM_GPU_ERR_LAUNCH;
 }
 
+int Operand::stage(const void* p, bool copy_in) {
+    const size_t bytes = matrix_extent(rows, cols, ld) * elem;
+    owned = g_stage_dev.acquire(bytes, &owned_bytes);
+    if (owned == nullptr) return ZYNUM_GPU_DECLINE_ALLOC;
+    bounce = g_stage_host.acquire(bytes, &bounce_bytes);
+    if (bounce == nullptr) return ZYNUM_GPU_DECLINE_ALLOC;
+    dev = owned;
+    if (copy_in) {
+        copy_columns(bounce, p, rows, cols, ld, elem);
+        hipError_t e = hipMemcpyAsync(owned, bounce, bytes, hipMemcpyHostToDevice, stream);
+        if (e != hipSuccess) { (void)hipGetLastError(); return ZYNUM_GPU_DECLINE_ALLOC; }
+    }
+    return ZYNUM_GPU_OK;
+}
+
 int Operand::bind_in(const void* p, int64_t r, int64_t c, int64_t ldx, size_t es, hipStream_t s) {
     rows = r; cols = c; ld = ldx; elem = es; host = p;
     if (r <= 0 || c <= 0) { dev = const_cast<void*>(p); return ZYNUM_GPU_OK; }
     if (is_device_pointer(p)) { dev = const_cast<void*>(p); return ZYNUM_GPU_OK; }
     staged = true;
-    size_t bytes = matrix_extent(r, c, ldx) * es;
-    if (hipMalloc(&owned, bytes) != hipSuccess) {
-        (void)hipGetLastError();
-        owned = nullptr;
-        return ZYNUM_GPU_DECLINE_ALLOC;
-    }
-    dev = owned;
-    hipError_t e = hipMemcpy2DAsync(owned, (size_t)ldx * es, p, (size_t)ldx * es, (size_t)r * es, (size_t)c,
-                                    hipMemcpyHostToDevice, s);
-    if (e != hipSuccess) { (void)hipGetLastError(); return ZYNUM_GPU_DECLINE_ALLOC; }
-    return ZYNUM_GPU_OK;
+    stream = s;
+    return stage(p, true);
 }
 
 int Operand::bind_out(void* p, int64_t r, int64_t c, int64_t ldx, size_t es, bool copy_in, hipStream_t s) {
@@ -94,33 +188,26 @@ int Operand::bind_out(void* p, int64_t r, int64_t c, int64_t ldx, size_t es, boo
     if (r <= 0 || c <= 0) { dev = p; return ZYNUM_GPU_OK; }
     if (is_device_pointer(p)) { dev = p; return ZYNUM_GPU_OK; }
     staged = true;
-    size_t bytes = matrix_extent(r, c, ldx) * es;
-    if (hipMalloc(&owned, bytes) != hipSuccess) {
-        (void)hipGetLastError();
-        owned = nullptr;
-        return ZYNUM_GPU_DECLINE_ALLOC;
-    }
-    dev = owned;
-    if (copy_in) {
-        hipError_t e = hipMemcpy2DAsync(owned, (size_t)ldx * es, p, (size_t)ldx * es, (size_t)r * es, (size_t)c,
-                                        hipMemcpyHostToDevice, s);
-        if (e != hipSuccess) { (void)hipGetLastError(); return ZYNUM_GPU_DECLINE_ALLOC; }
-    }
-    return ZYNUM_GPU_OK;
+    stream = s;
+    return stage(p, copy_in);
 }
 
 int Operand::finish(hipStream_t s) {
     if (!staged || host_out == nullptr) return ZYNUM_GPU_OK;
-    hipError_t e = hipMemcpy2DAsync(host_out, (size_t)ld * elem, owned, (size_t)ld * elem, (size_t)rows * elem,
-                                    (size_t)cols, hipMemcpyDeviceToHost, s);
+    hipError_t e = hipMemcpyAsync(bounce, owned, matrix_extent(rows, cols, ld) * elem, hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e == hipSuccess) copy_columns(host_out, bounce, rows, cols, ld, elem);  // the logical part only
     return status_from_hip(e);
 }
 
 void Operand::release() {
-    if (owned) {
-        (void)hipFree(owned);
+    if (owned || bounce) {
+        // idle already after a completed op; this also covers error returns
+        (void)hipStreamSynchronize(stream);
+        if (owned) g_stage_dev.release(owned, owned_bytes);
+        if (bounce) g_stage_host.release(bounce, bounce_bytes);
         owned = nullptr;
+        bounce = nullptr;
     }
 }
 
@@ -146,6 +233,8 @@ void* workspace_acquire(size_t bytes) {
 void workspace_release() { g_ws_mutex.unlock(); }
 
 void workspace_free_all() {
+    g_stage_dev.free_all();
+    g_stage_host.free_all();
     std::lock_guard<std::mutex> lock(g_ws_mutex);
     if (g_ws_ptr) {
         (void)hipDeviceSynchronize();

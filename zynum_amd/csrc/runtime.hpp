@@ -29,11 +29,16 @@ inline size_t matrix_extent(int64_t rows, int64_t cols, int64_t ld) {
 }
 
 // A device view of a caller matrix.  Host matrices are copied to a device
-// buffer (only the logical rows x cols part) and, for outputs, copied back by
-// finish().  Device matrices are used in place.
+// buffer through a pinned bounce buffer (one DMA transfer of the whole extent;
+// only the logical rows x cols part is read from or written to the caller)
+// and, for outputs, copied back by finish().  Device matrices are used in place.
 struct Operand {
     void* dev = nullptr;       // pointer handed to kernels
-    void* owned = nullptr;     // staging allocation (host operand only)
+    void* owned = nullptr;     // staging buffer (host operand only), from the staging cache
+    size_t owned_bytes = 0;
+    void* bounce = nullptr;    // pinned host bounce buffer of the same extent (host operand only)
+    size_t bounce_bytes = 0;
+    hipStream_t stream = nullptr;  // stream of the staging copies
     const void* host = nullptr;
     void* host_out = nullptr;
     int64_t rows = 0, cols = 0, ld = 0;
@@ -46,12 +51,15 @@ struct Operand {
     int bind_out(void* p, int64_t r, int64_t c, int64_t ldx, size_t es, bool copy_in, hipStream_t s);
     // Copy a staged output back to the host.
     int finish(hipStream_t s);
+    // Wait for the staging stream, then return the buffers to their caches.
     void release();
+    int stage(const void* p, bool copy_in);
     ~Operand() { release(); }
 };
 
 // Shared grow-only device workspace (one per process, current device).  The
 // lock is held by the caller for the whole stream-ordered use.
+// workspace_free_all also frees the cached staging buffers.
 void* workspace_acquire(size_t bytes);  // returns nullptr on allocation failure; holds the lock
 void workspace_release();
 void workspace_free_all();
