@@ -11,6 +11,7 @@ from __future__ import annotations
 
 import argparse
 import json
+import math
 import os
 import re
 import sys
@@ -43,6 +44,7 @@ def parse_args():
     p.add_argument("--no-grouped", action="store_true", help="skip the grouped uneven-GEMM leg")
     p.add_argument("--no-level3", action="store_true", help="skip the DTRSM/DSYRK leg")
     p.add_argument("--no-complex", action="store_true", help="skip the ZGEMM/CGEMM leg")
+    p.add_argument("--no-offload", action="store_true", help="skip the host-pointer offload leg")
     p.add_argument("--only-complex", action="store_true", help="headline plus the ZGEMM/CGEMM leg (profiler runs)")
     p.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU-baseline sample time")
     p.add_argument("--only-headline", action="store_true", help="plain FP64 leg only (for profiler runs)")
@@ -66,8 +68,9 @@ def main() -> int:
     args = parse_args()
     if args.only_headline:
         args.no_cpu = args.no_split = args.no_variants = args.no_grouped = args.no_level3 = args.no_complex = True
+        args.no_offload = True
     if args.only_complex:
-        args.no_cpu = args.no_split = args.no_variants = args.no_grouped = args.no_level3 = True
+        args.no_cpu = args.no_split = args.no_variants = args.no_grouped = args.no_level3 = args.no_offload = True
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print(json.dumps({"error": "this build runs on one GPU only; multi-GPU sharding is a design note, "
@@ -237,9 +240,20 @@ def main() -> int:
                 result["complex"] = {"error": str(exc)}
                 log(f"complex leg failed: {exc}")
 
+    # ---------------- host-pointer offload (GPU side) -----------------
+    if not args.no_offload:
+        try:
+            result["offload"] = offload_leg(torch, lib)
+        except Exception as exc:
+            result["offload"] = {"error": str(exc)}
+            log(f"offload leg failed: {exc}")
+
     # ---------------- CPU baseline -----------------
     if not args.no_cpu:
-        result["cpu_baseline"] = cpu_leg(n, ld, args.cpu_seconds)
+        edges = [] if args.no_offload or "error" in result["offload"] else OFFLOAD_EDGES
+        result["cpu_baseline"] = cpu_leg(n, ld, args.cpu_seconds, edges)
+        if edges:
+            offload_crossover(lib, result["offload"], result["cpu_baseline"]["by_edge"])
 
     print(json.dumps(result), flush=True)
     return 0
@@ -620,9 +634,60 @@ def complex_leg(torch, lib, n, stream, plain_tf):
     return out
 
 
-def cpu_leg(n, ld, target_s):
+OFFLOAD_EDGES = (64, 128, 192, 256, 384, 512, 768, 1024, 1536, 2048)
+
+
+def offload_leg(torch, lib):
+    """Host-pointer DGEMM through dgemm_ (A, B, C in host memory, staged by the
+    library) and the same call on device operands, against the offload model's
+    predictions, at square edges."""
+    from zynum_amd import _lib, autotune
+
+    rows = {}
+    for e in OFFLOAD_EDGES:
+        host = autotune.blas_call_ns(torch, lib, _lib.F64, e, e, e, True, 0.0, 5)
+        devn = autotune.blas_call_ns(torch, lib, _lib.F64, e, e, e, False, 0.0, 5)
+        ph = lib.zynum_gpu_offload_predict_ns(_lib.F64, e, e, e, 7, 0)
+        pd = lib.zynum_gpu_offload_predict_ns(_lib.F64, e, e, e, 0, 0)
+        rows[str(e)] = {"host_us": round(host / 1e3, 1), "device_us": round(devn / 1e3, 1),
+                        "predicted_host_us": round(ph / 1e3, 1), "predicted_device_us": round(pd / 1e3, 1)}
+        log(f"offload dgemm {e}^3: host {host / 1e3:.1f} us (model {ph / 1e3:.1f}), "
+            f"device {devn / 1e3:.1f} us (model {pd / 1e3:.1f})")
+    ratio = [r["predicted_host_us"] / r["host_us"] for r in rows.values()]
+    return {"workload": "DGEMM NN m=n=k=edge, alpha 1, beta 0, best of 5 synchronous dgemm_ calls; host: "
+                        "A, B, C in pageable host memory", "by_edge": rows,
+            "model_over_measured_host": [round(min(ratio), 3), round(max(ratio), 3)]}
+
+
+def offload_crossover(lib, off, cpu_by_edge):
+    """Where the measured host-pointer GPU call beats the CPU path, and whether
+    zynum_gpu_offload_choose, given the CPU rate at that edge, makes the same
+    choice."""
+    from zynum_amd import _lib
+
+    agree, measured_edge = 0, None
+    for e in OFFLOAD_EDGES:
+        row = off["by_edge"][str(e)]
+        cpu_us = cpu_by_edge[str(e)]["us"]
+        row["cpu_us"] = cpu_us
+        gpu_wins = row["host_us"] < cpu_us
+        rate = 2.0 * e ** 3 / (cpu_us * 1e3)  # GFLOP/s = FLOP per ns
+        row["choose"] = int(lib.zynum_gpu_offload_choose(_lib.F64, e, e, e, 7, 0, rate))
+        agree += row["choose"] == int(gpu_wins)
+        if gpu_wins and measured_edge is None:
+            measured_edge = e
+        elif not gpu_wins:
+            measured_edge = None
+    off["measured_crossover_edge"] = measured_edge
+    off["choice_agrees_with_measurement"] = f"{agree}/{len(OFFLOAD_EDGES)}"
+    log(f"offload crossover: GPU (host operands) wins from edge {measured_edge}; model choice agrees at "
+        f"{agree}/{len(OFFLOAD_EDGES)} edges")
+
+
+def cpu_leg(n, ld, target_s, edges=()):
     """The oracle (restated reference CPU path: packed-B 16x8 FMA tiles, column
-    panels over min(cores, 64) threads) on a column block of the same workload."""
+    panels over min(cores, 64) threads) on a column block of the same workload,
+    and at the square edges of the offload leg."""
     sys.path.insert(0, str(ROOT / "oracle"))
     import oracle
     from zynum_amd import synth
@@ -651,10 +716,23 @@ def cpu_leg(n, ld, target_s):
     except OSError:
         pass
     used = min(threads, (cols + 7) // 8)
+    by_edge = {}
+    for e in edges:
+        ae, be = synth.matrix(1, 0, e, e, e), synth.matrix(1, e * e, e, e, e)
+        best = math.inf
+        for _ in range(3 if e <= 1024 else 1):
+            ce = np.zeros(e * e)
+            t0 = oracle.now()
+            oracle.dgemm("N", "N", e, e, e, 1.0, ae, e, be, e, 0.0, ce, e)
+            best = min(best, oracle.now() - t0)
+        by_edge[str(e)] = {"us": round(best * 1e6, 1), "gflops": round(2.0 * e ** 3 / best / 1e9, 2)}
+    if by_edge:
+        log(f"cpu baseline by edge: {by_edge}")
     return {"value": round(rate, 5), "unit": "TFLOP/s", "cores": used, "kind": "port",
             "sample": f"DGEMM M={n} N={cols} K={n} (a {cols}-column block of the 8192^3 NN workload, ld={ld}), "
                       f"one call, {dt:.2f}s; restatement of Zynum's portable CPU path (reference not executable: "
-                      f"no Zig toolchain); host '{cpu_model}', {os.cpu_count()} logical CPUs"}
+                      f"no Zig toolchain); host '{cpu_model}', {os.cpu_count()} logical CPUs",
+            "by_edge": by_edge}
 
 
 if __name__ == "__main__":

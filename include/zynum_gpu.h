@@ -214,7 +214,7 @@ typedef struct zynum_gpu_gemm_request {
     uint8_t mode_param;  /* slices or moduli count; 0 picks the default */
     uint8_t flags;       /* ZYNUM_GPU_FLAG_* */
     uint8_t tile;        /* tile configuration of the GEMM core: 0 = tuned choice, i + 1 = configuration i
-                            (plain: FP64/FP32 MFMA tiles 0..10; INT8 modes: 0..13; BF16: 0..3) */
+                            (plain: FP64/FP32 MFMA tiles 0..11; INT8 modes: 0..13; BF16: 0..3) */
     uint8_t reserved1[4];
 } zynum_gpu_gemm_request; /* 104 bytes */
 
@@ -290,6 +290,30 @@ void zynum_gpu_debug_force_decline(int count);
 const char *zynum_gpu_status_string(int status);
 /* Library version string. */
 const char *zynum_gpu_version(void);
+
+/* Host-pointer offload crossover.  Replaces the work thresholds of the
+   reference's planner (src/blas/core/matrix_matrix/planner.zig:13,88-154,
+   which pick a CPU thread count from m*n*k) with a GPU-vs-CPU decision for a
+   planner that has this GPU tier: host operands are staged through cached
+   pinned bounce buffers, and the choice weighs that staging against the CPU
+   tier's time.  The model (measured device call times and staging rates,
+   written by zynum_amd/autotune.py) is host arithmetic only: these functions
+   touch no device and are safe to call on machines without a GPU.
+   host_mask: bit 0 A, bit 1 B, bit 2 C in host memory.  beta_nonzero: C is
+   read (staged in as well as out).  cpu_gflops: the CPU tier's rate for this
+   call, in GFLOP/s counted as 2mnk (8mnk complex). */
+/* Predicted wall nanoseconds of one synchronous GPU-tier call; 0 for m or
+   n = 0; negative for an invalid scalar, dimension or mask. */
+double zynum_gpu_offload_predict_ns(int scalar, int64_t m, int64_t n, int64_t k, int host_mask,
+                                    int beta_nonzero);
+/* 1: the GPU tier is predicted faster; 0: keep the call on the CPU (also for
+   m, n or k = 0); -ZYNUM_GPU_DECLINE_ARGUMENT for invalid arguments. */
+int zynum_gpu_offload_choose(int scalar, int64_t m, int64_t n, int64_t k, int host_mask, int beta_nonzero,
+                             double cpu_gflops);
+/* Smallest square edge e (m = n = k = e) the GPU tier takes at this CPU rate,
+   as a planner's precomputed threshold; -1 if none up to 65536;
+   -ZYNUM_GPU_DECLINE_ARGUMENT for invalid arguments. */
+int64_t zynum_gpu_offload_crossover(int scalar, int host_mask, int beta_nonzero, double cpu_gflops);
 
 /* Double-double reference product on the device (FP64 VALU, TwoProd/TwoSum):
    hi + lo ~= alpha*op(A)*op(B) + beta*C0 to about 2^-104 relative.  C0 may be

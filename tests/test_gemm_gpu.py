@@ -244,7 +244,7 @@ def test_8192_padded_integer_sampled(torch_cuda, dtype, ta, tb, beta):
     assert torch.all(Cm[:, n:] == -999.0)
 
 
-PLAIN_TILES = 11
+PLAIN_TILES = 12
 
 
 @pytest.mark.parametrize("tile", range(1, PLAIN_TILES + 1))

@@ -131,6 +131,9 @@ SIGNATURES = {
     "zynum_gpu_debug_force_decline": (None, [C.c_int]),
     "zynum_gpu_status_string": (C.c_char_p, [C.c_int]),
     "zynum_gpu_version": (C.c_char_p, []),
+    "zynum_gpu_offload_predict_ns": (C.c_double, [C.c_int, C.c_int64, C.c_int64, C.c_int64, C.c_int, C.c_int]),
+    "zynum_gpu_offload_choose": (C.c_int, [C.c_int, C.c_int64, C.c_int64, C.c_int64, C.c_int, C.c_int, C.c_double]),
+    "zynum_gpu_offload_crossover": (C.c_int64, [C.c_int, C.c_int, C.c_int, C.c_double]),
     "zynum_gpu_dd_gemm": (C.c_int, [C.c_char, C.c_char, C.c_int, C.c_int, C.c_int, C.c_double, C.c_void_p, C.c_int,
                                     C.c_void_p, C.c_int, C.c_double, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p,
                                     C.c_int, C.c_void_p]),

@@ -32,12 +32,13 @@ enum TileConfig : int {
     kTile256x128w8 = 8,   // 512 threads, 4x2 waves of 64x64, one workgroup per CU
     kTile128x256w8 = 9,   // 512 threads, 2x4 waves of 64x64, one workgroup per CU
     kTile128x128w8k8 = 10,  // as 6 with BK 8 (half the LDS: more workgroups per CU)
-    kTileConfigs = 11
+    kTile128x128w16 = 11,   // 1024 threads, 4x4 waves of 32x32, BK 16, 8 waves per SIMD
+    kTileConfigs = 12
 };
 
 // Tile choice for one GEMM shape from the measured tuning table
 // (tune_table.inc, written by zynum_amd/autotune.py).
-int pick_tile(int m, int n, int k, int elem_bytes);
+int pick_tile(int m, int n, int k, int elem_bytes, bool ta, bool tb);
 
 // Launch the plain MFMA GEMM (FP64 or FP32). Returns a hipError_t.
 template <typename T>

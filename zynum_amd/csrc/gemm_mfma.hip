@@ -103,7 +103,7 @@ template <typename T>
 hipError_t gemm_launch(const GemmDesc<T>& d, hipStream_t stream, int tile) {
     if (d.m <= 0 || d.n <= 0) return hipSuccess;
     if (d.k <= 0 || d.alpha == T(0)) return scale_launch<T>(d.m, d.n, d.beta, d.c, d.ldc, stream);
-    if (tile == kTileAuto) tile = pick_tile(d.m, d.n, d.k, (int)sizeof(T));
+    if (tile == kTileAuto) tile = pick_tile(d.m, d.n, d.k, (int)sizeof(T), d.ta, d.tb);
     switch (tile) {
         case kTile64x64: return launch_shape<T, 64, 64, 16, 2, 2, 0>(d, stream);
         case kTile128x128: return launch_shape<T, 128, 128, 16, 2, 2, 1>(d, stream);
@@ -116,6 +116,7 @@ hipError_t gemm_launch(const GemmDesc<T>& d, hipStream_t stream, int tile) {
         case kTile256x128w8: return launch_shape<T, 256, 128, 16, 4, 2, 1>(d, stream);
         case kTile128x256w8: return launch_shape<T, 128, 256, 16, 2, 4, 1>(d, stream);
         case kTile128x128w8k8: return launch_shape<T, 128, 128, 8, 4, 2, 2>(d, stream);
+        case kTile128x128w16: return launch_shape<T, 128, 128, 16, 4, 4, 4>(d, stream);
         default: return hipErrorInvalidValue;
     }
 }
