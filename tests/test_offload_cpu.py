@@ -51,7 +51,7 @@ def test_device_times_at_and_between_measured_points(lib):
         for (_, w1, t1), (_, w2, t2) in zip(mine, mine[1:]):
             e1, e2 = round(w1 ** (1 / 3)), round(w2 ** (1 / 3))
             got = lib.zynum_gpu_offload_predict_ns(s, e1, e1, e2, 0, 0)  # work strictly between the two
-            assert min(t1, t2) <= got <= max(t1, t2)
+            assert min(t1, t2) * (1 - 1e-12) <= got <= max(t1, t2) * (1 + 1e-12)
         # below the first point: the first point's time; beyond the last: linear in work
         _, w0, t0 = mine[0]
         assert lib.zynum_gpu_offload_predict_ns(s, 1, 1, 1, 0, 0) == pytest.approx(t0)

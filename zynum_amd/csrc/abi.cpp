@@ -158,7 +158,18 @@ int gemm_op(bool ta, bool tb, int m, int n, int k, T alpha, const T* a, int64_t 
         d.lda = lda; d.ldb = ldb; d.ldc = ldc;
         d.alpha = alpha; d.beta = beta;
         d.a = da; d.b = db; d.c = dc;
-        st = status_from_hip(gemm_launch<T>(d, s, tile));
+        int kchunk = 0;
+        const int parts = (reads_ab && tile == kTileAuto) ? gemm_splitk_plan(m, n, k, &kchunk) : 1;
+        size_t got = 0;
+        void* part = parts > 1 ? scratch_acquire((size_t)parts * m * n * sizeof(T), &got) : nullptr;
+        if (part != nullptr) {
+            hipError_t e = gemm_splitk_launch<T>(d, s, static_cast<T*>(part), parts, kchunk);
+            const hipError_t e2 = hipStreamSynchronize(s);  // the partial buffer goes back to the cache
+            st = status_from_hip(e != hipSuccess ? e : e2);
+            if (e == hipSuccess && e2 == hipSuccess) scratch_release(part, got);  // else leaked, never reused in flight
+        } else {
+            st = status_from_hip(gemm_launch<T>(d, s, tile));
+        }
     } else if (mode == ZYNUM_GPU_MODE_DD) {
         if constexpr (sizeof(T) == 8) st = dd_dgemm(ta, tb, m, n, k, alpha, da, lda, db, ldb, beta, dc, ldc, s);
     } else {

@@ -274,7 +274,18 @@ int cgemm_device(char ta, char tb, int m, int n, int k, T alpha_re, T alpha_im, 
         d.a = a2;
         d.b = breal;
         d.c = c + 2 * i0;
-        e = gemm_launch<T>(d, s, tile);
+        int kchunk = 0;  // few output tiles and a long K: split K (partials from the scratch cache)
+        const int parts = tile == kTileAuto ? gemm_splitk_plan(d.m, d.n, d.k, &kchunk) : 1;
+        size_t got = 0;
+        void* part = parts > 1 ? scratch_acquire((size_t)parts * d.m * d.n * sizeof(T), &got) : nullptr;
+        if (part != nullptr) {
+            e = gemm_splitk_launch<T>(d, s, static_cast<T*>(part), parts, kchunk);
+            const hipError_t e2 = hipStreamSynchronize(s);
+            if (e == hipSuccess) e = e2;
+            if (e == hipSuccess) scratch_release(part, got);
+        } else {
+            e = gemm_launch<T>(d, s, tile);
+        }
     }
     int st = status_from_hip(e);
     if (st == ZYNUM_GPU_OK) st = status_from_hip(hipStreamSynchronize(s));  // workspace reuse is host-ordered

@@ -44,6 +44,15 @@ int pick_tile(int m, int n, int k, int elem_bytes, bool ta, bool tb);
 template <typename T>
 hipError_t gemm_launch(const GemmDesc<T>& d, hipStream_t stream, int tile = kTileAuto);
 
+// Split-K for few output tiles and a long K: the number of K ranges (1 = do
+// not split) and, through kchunk, their depth.
+int gemm_splitk_plan(int m, int n, int k, int* kchunk);
+// The split-K GEMM: parts x m x n partial products in `part` (device memory
+// the caller provides and keeps until the stream has passed the launch), then
+// C = alpha * (sum of the parts in increasing order) + beta * C.
+template <typename T>
+hipError_t gemm_splitk_launch(const GemmDesc<T>& d, hipStream_t stream, T* part, int parts, int kchunk);
+
 // C = beta * C (beta == 0 writes exact zeros), the k == 0 / alpha == 0 path.
 template <typename T>
 hipError_t scale_launch(int m, int n, T beta, T* c, int64_t ldc, hipStream_t stream);

@@ -18,6 +18,8 @@
 //     with zero fill; interior tiles run unguarded;
 //   * tile ids are remapped so each of the 8 XCDs gets a contiguous block of
 //     tiles in grouped (super-row) order, keeping shared A/B panels in its L2.
+#include <algorithm>
+
 #include "common.hpp"
 #include "gemm.hpp"
 #include "gemm_tile.hpp"
@@ -38,6 +40,40 @@ void gemm_mfma_kernel(GemmArgs<T> p) {
     int tm, tn;
     grouped_tile(xcd_remap(blockIdx.x, total), p.tiles_m, p.tiles_n, p.group_m, tm, tn);
     gemm_tile<T, CF, TA, TB, VEC>(p, tm * CF::BM, tn * CF::BN, lds_a, lds_b);
+}
+
+// Split-K form for shapes with few output tiles and a long K: blockIdx.y
+// selects a K range [k0, k0 + kchunk) of the product, written unscaled
+// (alpha 1, beta 0: the accumulator itself) to its own slice of the partial
+// buffer; splitk_reduce_kernel sums the slices in increasing order and applies
+// the alpha/beta epilogue.
+template <typename T, class CF, bool TA, bool TB, int VEC>
+__global__ __launch_bounds__(CF::NT) __attribute__((amdgpu_waves_per_eu(CF::OCC > 0 ? CF::OCC : 1)))
+void gemm_splitk_kernel(GemmArgs<T> p, int kchunk, int64_t part_stride) {
+    __shared__ __attribute__((aligned(16))) T lds_a[tile_lds_a<T, CF>()];
+    __shared__ __attribute__((aligned(16))) T lds_b[tile_lds_b<T, CF>()];
+    const int k0 = (int)blockIdx.y * kchunk;
+    GemmArgs<T> q = p;
+    q.k = min(kchunk, p.k - k0);
+    q.a = TA ? p.a + k0 : p.a + (int64_t)k0 * p.lda;
+    q.b = TB ? p.b + (int64_t)k0 * p.ldb : p.b + k0;
+    q.c = p.c + (int64_t)blockIdx.y * part_stride;
+    const int total = p.tiles_m * p.tiles_n;
+    int tm, tn;
+    grouped_tile(xcd_remap(blockIdx.x, total), p.tiles_m, p.tiles_n, p.group_m, tm, tn);
+    gemm_tile<T, CF, TA, TB, VEC>(q, tm * CF::BM, tn * CF::BN, lds_a, lds_b);
+}
+
+template <typename T>
+__global__ void splitk_reduce_kernel(int m, int n, int parts, const T* __restrict__ part, int64_t part_stride,
+                                     T alpha, T beta, T* __restrict__ c, int64_t ldc) {
+    const int64_t total = (int64_t)m * n;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+        T acc = part[e];
+        for (int s = 1; s < parts; ++s) acc += part[s * part_stride + e];
+        T* cp = c + (e / m) * ldc + e % m;
+        *cp = epilogue(alpha, beta, acc, cp);
+    }
 }
 
 template <typename T>
@@ -121,6 +157,59 @@ hipError_t gemm_launch(const GemmDesc<T>& d, hipStream_t stream, int tile) {
     }
 }
 
+// Split-K plan on the 64x64 tile: split when fewer than 128 tiles would run
+// and K >= 1024, into enough K ranges (each a multiple of 32 and at least
+// 256 deep, at most 64 of them) to give about 512 workgroups.
+int gemm_splitk_plan(int m, int n, int k, int* kchunk) {
+    const int64_t tiles = (int64_t)((m + 63) / 64) * ((n + 63) / 64);
+    if (tiles >= 128 || k < 1024) return 1;
+    int64_t parts = std::min<int64_t>({(512 + tiles - 1) / tiles, (int64_t)k / 256, 64});
+    if (parts < 2) return 1;
+    int64_t kc = ((k + parts - 1) / parts + 31) / 32 * 32;
+    parts = (k + kc - 1) / kc;
+    if (parts < 2) return 1;
+    *kchunk = (int)kc;
+    return (int)parts;
+}
+
+template <typename T>
+hipError_t gemm_splitk_launch(const GemmDesc<T>& d, hipStream_t stream, T* part, int parts, int kchunk) {
+    using CF = TileCfg<64, 64, (sizeof(T) == 8) ? 16 : 32, 2, 2, 0>;
+    GemmArgs<T> p;
+    p.m = d.m; p.n = d.n; p.k = d.k;
+    p.lda = d.lda; p.ldb = d.ldb; p.ldc = d.m;
+    p.alpha = T(1); p.beta = T(0);
+    p.a = d.a; p.b = d.b; p.c = part;
+    p.tiles_m = (d.m + CF::BM - 1) / CF::BM;
+    p.tiles_n = (d.n + CF::BN - 1) / CF::BN;
+    p.group_m = ZG_GROUP_M;
+    const int64_t stride = (int64_t)d.m * d.n;
+    constexpr int V = 16 / sizeof(T);
+    const bool vec = ((reinterpret_cast<uintptr_t>(d.a) & 15) == 0) && ((reinterpret_cast<uintptr_t>(d.b) & 15) == 0) &&
+                     (d.lda % V == 0) && (d.ldb % V == 0) && (kchunk % V == 0);
+    const dim3 grid(p.tiles_m * p.tiles_n, parts);
+#define ZG_SPLITK(TA, TB, VV) gemm_splitk_kernel<T, CF, TA, TB, VV><<<grid, CF::NT, 0, stream>>>(p, kchunk, stride)
+    if (vec) {
+        if (!d.ta && !d.tb) ZG_SPLITK(false, false, V);
+        else if (!d.ta && d.tb) ZG_SPLITK(false, true, V);
+        else if (d.ta && !d.tb) ZG_SPLITK(true, false, V);
+        else ZG_SPLITK(true, true, V);
+    } else {
+        if (!d.ta && !d.tb) ZG_SPLITK(false, false, 1);
+        else if (!d.ta && d.tb) ZG_SPLITK(false, true, 1);
+        else if (d.ta && !d.tb) ZG_SPLITK(true, false, 1);
+        else ZG_SPLITK(true, true, 1);
+    }
+#undef ZG_SPLITK
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    int blocks = (int)std::min<int64_t>(4096, (stride + 255) / 256);
+    splitk_reduce_kernel<T><<<blocks, 256, 0, stream>>>(d.m, d.n, parts, part, stride, d.alpha, d.beta, d.c, d.ldc);
+    return hipGetLastError();
+}
+
+template hipError_t gemm_splitk_launch<double>(const GemmDesc<double>&, hipStream_t, double*, int, int);
+template hipError_t gemm_splitk_launch<float>(const GemmDesc<float>&, hipStream_t, float*, int, int);
 template hipError_t gemm_launch<double>(const GemmDesc<double>&, hipStream_t, int);
 template hipError_t gemm_launch<float>(const GemmDesc<float>&, hipStream_t, int);
 template hipError_t scale_launch<double>(int, int, double, double*, int64_t, hipStream_t);

@@ -3,9 +3,15 @@
 // sequence of kernel launches on the current HIP device.
 #include "runtime.hpp"
 
-#include <atomic>
-#include <mutex>
+#include <stdlib.h>
 #include <string.h>
+
+#include <algorithm>
+#include <atomic>
+#include <condition_variable>
+#include <functional>
+#include <mutex>
+#include <thread>
 #include <vector>
 
 namespace zg {
@@ -95,16 +101,137 @@ class BufCache {
 BufCache g_stage_dev(false, size_t(8) << 30);
 BufCache g_stage_host(true, size_t(2) << 30);
 
+// Host threads for the copies between caller memory and the pinned bounce
+// buffers: one core copies pageable memory at about half the rate of the DMA
+// engine, several together at more than it.  The caller takes parts too.
+// Workers are created on first use and parked on a condition variable; they
+// are detached and end with the process.
+class CopyPool {
+  public:
+    static CopyPool& get() {
+        static CopyPool* pool = new CopyPool();
+        return *pool;
+    }
+
+    int threads() const { return nthreads_; }
+
+    // Run fn(0) .. fn(parts - 1), spread over the workers and the caller.
+    void run(int parts, const std::function<void(int)>& fn) {
+        if (parts <= 1 || nthreads_ <= 1) {
+            for (int i = 0; i < parts; ++i) fn(i);
+            return;
+        }
+        std::lock_guard<std::mutex> serial(run_mu_);  // one job at a time
+        {
+            std::lock_guard<std::mutex> lock(mu_);
+            fn_ = &fn;
+            parts_ = parts;
+            next_.store(0);
+            pending_ = (int)workers_.size();
+            ++generation_;
+        }
+        cv_.notify_all();
+        work();
+        std::unique_lock<std::mutex> lock(mu_);
+        done_cv_.wait(lock, [this] { return pending_ == 0; });
+        fn_ = nullptr;
+    }
+
+  private:
+    CopyPool() {
+        int n = 8;
+        const unsigned hw = std::thread::hardware_concurrency();
+        if (hw > 0) n = std::min(n, (int)std::max(1u, hw / 2));
+        if (const char* env = getenv("ZYNUM_GPU_COPY_THREADS")) n = std::max(1, std::min(64, atoi(env)));
+        nthreads_ = n;
+        for (int i = 1; i < n; ++i) {
+            workers_.emplace_back([this] { loop(); });
+            workers_.back().detach();
+        }
+    }
+
+    void work() {
+        for (int i = next_.fetch_add(1); i < parts_; i = next_.fetch_add(1)) (*fn_)(i);
+    }
+
+    void loop() {
+        uint64_t seen = 0;
+        for (;;) {
+            {
+                std::unique_lock<std::mutex> lock(mu_);
+                cv_.wait(lock, [&] { return generation_ != seen; });
+                seen = generation_;
+            }
+            work();
+            std::lock_guard<std::mutex> lock(mu_);
+            if (--pending_ == 0) done_cv_.notify_one();
+        }
+    }
+
+    int nthreads_ = 1;
+    std::vector<std::thread> workers_;
+    std::mutex run_mu_, mu_;
+    std::condition_variable cv_, done_cv_;
+    const std::function<void(int)>* fn_ = nullptr;
+    int parts_ = 0;
+    std::atomic<int> next_{0};
+    int pending_ = 0;
+    uint64_t generation_ = 0;
+};
+
 // Copy the logical rows x cols part of a column-major matrix between two
-// buffers with the same leading dimension (one memcpy when there is no padding).
+// buffers with the same leading dimension, over the copy threads when it is
+// large (pieces of at least 1 MiB; one memcpy per piece without padding).
 void copy_columns(void* dst, const void* src, int64_t rows, int64_t cols, int64_t ld, size_t es) {
+    const size_t total = (size_t)(rows * cols) * es;
+    char* d = static_cast<char*>(dst);
+    const char* s = static_cast<const char*>(src);
+    CopyPool& pool = CopyPool::get();
+    const int parts = (int)std::min<size_t>((size_t)pool.threads() * 2, std::max<size_t>(1, total >> 20));
     if (rows == ld) {
-        memcpy(dst, src, (size_t)(rows * cols) * es);
+        const size_t piece = (total + parts - 1) / parts;
+        pool.run(parts, [&](int i) {
+            const size_t off = (size_t)i * piece;
+            if (off < total) memcpy(d + off, s + off, std::min(piece, total - off));
+        });
         return;
     }
-    for (int64_t j = 0; j < cols; ++j)
-        memcpy(static_cast<char*>(dst) + (size_t)(j * ld) * es, static_cast<const char*>(src) + (size_t)(j * ld) * es,
-               (size_t)rows * es);
+    const int64_t per = (cols + parts - 1) / parts;
+    pool.run(parts, [&](int i) {
+        for (int64_t j = (int64_t)i * per; j < std::min(cols, (int64_t)(i + 1) * per); ++j)
+            memcpy(d + (size_t)(j * ld) * es, s + (size_t)(j * ld) * es, (size_t)rows * es);
+    });
+}
+
+// Column ranges of a staged matrix are moved in chunks so that the host copy
+// of one chunk overlaps the DMA transfer of the previous one (the bounce
+// buffer holds the whole extent, so no chunk is overwritten in flight).
+int64_t chunk_columns(int64_t rows, int64_t cols, int64_t ld, size_t es) {
+    const size_t bytes = matrix_extent(rows, cols, ld) * es;
+    const size_t target = std::min<size_t>(size_t(32) << 20, std::max<size_t>(size_t(4) << 20, bytes / 4));
+    const size_t col_bytes = (size_t)ld * es;
+    return std::max<int64_t>(1, std::min<int64_t>(cols, (int64_t)(target / col_bytes)));
+}
+
+// Bytes of columns [j0, j1) of the extent (the last column ends at row `rows`).
+size_t chunk_bytes(int64_t rows, int64_t cols, int64_t ld, size_t es, int64_t j0, int64_t j1) {
+    return (size_t)((j1 == cols ? (j1 - j0 - 1) * ld + rows : (j1 - j0) * ld)) * es;
+}
+
+// Events marking the end of each device-to-host chunk (grown on demand, kept
+// per host thread).
+thread_local std::vector<hipEvent_t> t_chunk_events;
+
+hipEvent_t chunk_event(size_t i) {
+    while (t_chunk_events.size() <= i) {
+        hipEvent_t e = nullptr;
+        if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
+            (void)hipGetLastError();
+            return nullptr;
+        }
+        t_chunk_events.push_back(e);
+    }
+    return t_chunk_events[i];
 }
 }  // namespace
 
@@ -166,9 +293,14 @@ int Operand::stage(const void* p, bool copy_in) {
     bounce = g_stage_host.acquire(bytes, &bounce_bytes);
     if (bounce == nullptr) return ZYNUM_GPU_DECLINE_ALLOC;
     dev = owned;
-    if (copy_in) {
-        copy_columns(bounce, p, rows, cols, ld, elem);
-        hipError_t e = hipMemcpyAsync(owned, bounce, bytes, hipMemcpyHostToDevice, stream);
+    if (!copy_in) return ZYNUM_GPU_OK;
+    const int64_t step = chunk_columns(rows, cols, ld, elem);
+    for (int64_t j0 = 0; j0 < cols; j0 += step) {
+        const int64_t j1 = std::min(cols, j0 + step);
+        const size_t off = (size_t)(j0 * ld) * elem;
+        copy_columns(static_cast<char*>(bounce) + off, static_cast<const char*>(p) + off, rows, j1 - j0, ld, elem);
+        hipError_t e = hipMemcpyAsync(static_cast<char*>(owned) + off, static_cast<char*>(bounce) + off,
+                                      chunk_bytes(rows, cols, ld, elem, j0, j1), hipMemcpyHostToDevice, stream);
         if (e != hipSuccess) { (void)hipGetLastError(); return ZYNUM_GPU_DECLINE_ALLOC; }
     }
     return ZYNUM_GPU_OK;
@@ -194,9 +326,31 @@ int Operand::bind_out(void* p, int64_t r, int64_t c, int64_t ldx, size_t es, boo
 
 int Operand::finish(hipStream_t s) {
     if (!staged || host_out == nullptr) return ZYNUM_GPU_OK;
-    hipError_t e = hipMemcpyAsync(bounce, owned, matrix_extent(rows, cols, ld) * elem, hipMemcpyDeviceToHost, s);
-    if (e == hipSuccess) e = hipStreamSynchronize(s);
-    if (e == hipSuccess) copy_columns(host_out, bounce, rows, cols, ld, elem);  // the logical part only
+    // Queue every chunk's transfer with an event behind it, then copy each
+    // chunk's logical part to the caller as soon as its transfer is done.
+    const int64_t step = chunk_columns(rows, cols, ld, elem);
+    hipError_t e = hipSuccess;
+    size_t nchunks = 0;
+    for (int64_t j0 = 0; j0 < cols && e == hipSuccess; j0 += step, ++nchunks) {
+        const int64_t j1 = std::min(cols, j0 + step);
+        const size_t off = (size_t)(j0 * ld) * elem;
+        e = hipMemcpyAsync(static_cast<char*>(bounce) + off, static_cast<char*>(owned) + off,
+                           chunk_bytes(rows, cols, ld, elem, j0, j1), hipMemcpyDeviceToHost, s);
+        hipEvent_t ev = chunk_event(nchunks);
+        if (e == hipSuccess) e = ev ? hipEventRecord(ev, s) : hipErrorOutOfMemory;
+    }
+    if (e == hipSuccess) {
+        size_t i = 0;
+        for (int64_t j0 = 0; j0 < cols && e == hipSuccess; j0 += step, ++i) {
+            const int64_t j1 = std::min(cols, j0 + step);
+            const size_t off = (size_t)(j0 * ld) * elem;
+            e = hipEventSynchronize(t_chunk_events[i]);
+            if (e == hipSuccess)  // the logical part only
+                copy_columns(static_cast<char*>(host_out) + off, static_cast<char*>(bounce) + off, rows, j1 - j0, ld,
+                             elem);
+        }
+    }
+    if (e != hipSuccess) (void)hipStreamSynchronize(s);
     return status_from_hip(e);
 }
 
@@ -210,6 +364,10 @@ void Operand::release() {
         bounce = nullptr;
     }
 }
+
+void* scratch_acquire(size_t bytes, size_t* got) { return g_stage_dev.acquire(bytes, got); }
+
+void scratch_release(void* p, size_t bytes) { g_stage_dev.release(p, bytes); }
 
 void* workspace_acquire(size_t bytes) {
     g_ws_mutex.lock();

@@ -64,6 +64,11 @@ void* workspace_acquire(size_t bytes);  // returns nullptr on allocation failure
 void workspace_release();
 void workspace_free_all();
 
+// Device scratch memory from the staging-buffer cache (no lock held, unlike
+// the workspace).  Release only after the stream that used it is synchronised.
+void* scratch_acquire(size_t bytes, size_t* got);  // nullptr on allocation failure
+void scratch_release(void* p, size_t bytes);
+
 int status_from_hip(hipError_t e);
 
 }  // namespace zg
