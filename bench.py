@@ -276,11 +276,13 @@ def split_leg(torch, lib, A, B, n, ld, stream, plain_tf):
     _lib.check(st, "dd reference")
     flop = 2.0 * n ** 3
 
-    def run(mode, param, reps):
+    def run(mode, param, reps, warm=1):
         C = torch.zeros(ld * n, dtype=torch.float64, device=dev)
-        st = blas.gemm_request(False, False, n, n, n, 1.0, Aw.data_ptr(), ld, Bw.data_ptr(), ld, 0.0, C.data_ptr(),
-                               ld, _lib.F64, stream=sh, mode=mode, mode_param=param, flags=_lib.FLAG_ASYNC)
-        _lib.check(st, f"mode {mode}")
+        for _ in range(warm):  # untimed
+            st = blas.gemm_request(False, False, n, n, n, 1.0, Aw.data_ptr(), ld, Bw.data_ptr(), ld, 0.0,
+                                   C.data_ptr(), ld, _lib.F64, stream=sh, mode=mode, mode_param=param,
+                                   flags=_lib.FLAG_ASYNC)
+            _lib.check(st, f"mode {mode}")
         s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         s.record(stream)
         for _ in range(reps):
@@ -305,7 +307,7 @@ def split_leg(torch, lib, A, B, n, ld, stream, plain_tf):
            "plain_fp64": plain}
     best = None
     for moduli in (13, 14, 15, 16):
-        r = run(_lib.MODE_OZAKI_CRT, moduli, 5)
+        r = run(_lib.MODE_OZAKI_CRT, moduli, 5, warm=3)
         r["int8_gemms"] = moduli
         r["int8_tops_executed"] = round(moduli * flop / (r["ms"] * 1e-3) / 1e12, 1)
         out[f"crt_{moduli}"] = r
@@ -718,6 +720,8 @@ def cpu_leg(n, ld, target_s, edges=()):
     used = min(threads, (cols + 7) // 8)
     by_edge = {}
     for e in edges:
+        # the reference planner runs work below 192^3 on one thread (planner.zig:13,135-144)
+        oracle.set_threads(1 if e ** 3 < 192 ** 3 else 0)
         ae, be = synth.matrix(1, 0, e, e, e), synth.matrix(1, e * e, e, e, e)
         best = math.inf
         for _ in range(3 if e <= 1024 else 1):
@@ -725,7 +729,9 @@ def cpu_leg(n, ld, target_s, edges=()):
             t0 = oracle.now()
             oracle.dgemm("N", "N", e, e, e, 1.0, ae, e, be, e, 0.0, ce, e)
             best = min(best, oracle.now() - t0)
-        by_edge[str(e)] = {"us": round(best * 1e6, 1), "gflops": round(2.0 * e ** 3 / best / 1e9, 2)}
+        by_edge[str(e)] = {"us": round(best * 1e6, 1), "gflops": round(2.0 * e ** 3 / best / 1e9, 2),
+                           "threads": oracle.threads()}
+    oracle.set_threads(0)
     if by_edge:
         log(f"cpu baseline by edge: {by_edge}")
     return {"value": round(rate, 5), "unit": "TFLOP/s", "cores": used, "kind": "port",

@@ -517,6 +517,11 @@ __device__ __forceinline__ void i8_store(const I8Args& p, const I8Out& out, int 
 // every product of two slices is an integer of at most 2^12 in magnitude, so
 // FP32 sums of up to 4096 of them are exact; the FP32 accumulators are added
 // into FP64 ones every kFlush steps (BK/2 products per step) and cleared.
+// Padding (bytes) of each LDS row of the register-staged INT8 kernel; a build knob for tuning.
+#ifndef ZG_I8_LDS_PAD
+#define ZG_I8_LDS_PAD 16
+#endif
+
 template <int BM, int BN, int BK, int WM, int WN, int OCC, class MF>
 __global__ __launch_bounds__(64 * WM * WN) __attribute__((amdgpu_waves_per_eu(OCC)))
 void i8_gemm_kernel(I8Args p) {
@@ -525,7 +530,7 @@ void i8_gemm_kernel(I8Args p) {
     constexpr int kFlush = 8192 / BK;
     typedef typename MF::acc_t acc_t;
     constexpr int NT = 64 * WM * WN;
-    constexpr int RS = BK + 16;      // padded LDS row (bytes): 16-byte reads of consecutive rows hit distinct banks
+    constexpr int RS = BK + ZG_I8_LDS_PAD;  // padded LDS row (bytes): 16-byte reads of consecutive rows hit distinct banks
     constexpr int CPR = BK / 16;     // 16-byte chunks per row
     constexpr int WTM = BM / WM, WTN = BN / WN;
     constexpr int FM = WTM / MS, FN = WTN / MS;
@@ -923,65 +928,80 @@ struct CrtArgs {
 // One thread per 8 consecutive rows of one column: the residue bytes of all
 // moduli are loaded first (one 64-bit load per modulus, all in flight), then
 // summed into the CRT digits.  Rows m..ldu-1 of a residue plane are padding.
+// The products go through LDS so that the block's 2048 rows of C are read (for
+// beta) and written with consecutive lanes on consecutive rows.
 constexpr int kCrtRows = 8;
-__global__ __launch_bounds__(256) void crt_reconstruct_kernel(CrtArgs p) {
-    const int i0 = (blockIdx.x * 256 + threadIdx.x) * kCrtRows;
+constexpr int kCrtThreads = 256;
+__global__ __launch_bounds__(kCrtThreads) void crt_reconstruct_kernel(CrtArgs p) {
+    __shared__ double prods[kCrtThreads * (kCrtRows + 1)];  // row stride 9: conflict-free 8-byte writes
+    const int base = blockIdx.x * kCrtThreads * kCrtRows;
+    const int t = threadIdx.x;
+    const int i0 = base + t * kCrtRows;
     const int j = blockIdx.y;
-    if (i0 >= p.m) return;
-    const int nd = p.cc.ndig, nm = p.cc.nmod;
-    const uint8_t* u = p.u + (int64_t)j * p.ldu + i0;
-    uint2 w[kMaxModuli];
+    const int sb = p.sig_b[j];
+    if (i0 < p.m) {
+        const int nd = p.cc.ndig, nm = p.cc.nmod;
+        const uint8_t* u = p.u + (int64_t)j * p.ldu + i0;
+        uint2 w[kMaxModuli];
 #pragma unroll
-    for (int l = 0; l < kMaxModuli; ++l)
-        if (l < nm) w[l] = *reinterpret_cast<const uint2*>(u + (int64_t)l * p.u_plane);
-    double S[kCrtRows][kMaxDigits] = {};
+        for (int l = 0; l < kMaxModuli; ++l)
+            if (l < nm) w[l] = *reinterpret_cast<const uint2*>(u + (int64_t)l * p.u_plane);
+        double S[kCrtRows][kMaxDigits] = {};
 #pragma unroll
-    for (int l = 0; l < kMaxModuli; ++l) {
-        if (l < nm) {
+        for (int l = 0; l < kMaxModuli; ++l) {
+            if (l < nm) {
 #pragma unroll
-            for (int r = 0; r < kCrtRows; ++r) {
-                const uint32_t word = (r < 4) ? w[l].x : w[l].y;
-                const double ul = (double)((word >> (8 * (r & 3))) & 0xffu);
+                for (int r = 0; r < kCrtRows; ++r) {
+                    const uint32_t word = (r < 4) ? w[l].x : w[l].y;
+                    const double ul = (double)((word >> (8 * (r & 3))) & 0xffu);
 #pragma unroll
-                for (int d = 0; d < kMaxDigits; ++d)
-                    if (d < nd) S[r][d] = fma(p.cc.h[l][d], ul, S[r][d]);  // exact: integer < 2^53
+                    for (int d = 0; d < kMaxDigits; ++d)
+                        if (d < nd) S[r][d] = fma(p.cc.h[l][d], ul, S[r][d]);  // exact: integer < 2^53
+                }
             }
+        }
+#pragma unroll
+        for (int r = 0; r < kCrtRows; ++r) {
+            const int i = i0 + r;
+            double xq = 0.0;
+#pragma unroll
+            for (int d = 0; d < kMaxDigits; ++d)
+                if (d < nd) xq = fma(S[r][d], p.cc.inv_p_dig[d], xq);
+            const double q = rint(xq);
+            // C' = sum_d D_d 2^(40 (nd-1-d)) with D_d = S_d - q P_d (exact, |D_d| < 2^53).
+            // Normalize to balanced 40-bit digits with exact carries, so that every
+            // lower part is below half an ulp of what is added on top of it; the sum
+            // is then within one rounding of C' and exact when C' is representable.
+            double D[kMaxDigits];
+#pragma unroll
+            for (int d = 0; d < kMaxDigits; ++d) D[d] = (d < nd) ? fma(-q, p.cc.p[d], S[r][d]) : 0.0;
+#pragma unroll
+            for (int d = kMaxDigits - 1; d >= 1; --d) {
+                if (d < nd) {
+                    const double carry = rint(D[d] * 0x1p-40);
+                    D[d] = fma(-carry, 0x1p40, D[d]);  // exact, |D_d| <= 2^39
+                    D[d - 1] += carry;                 // exact, integers well below 2^53
+                }
+            }
+            double cp = 0.0;
+#pragma unroll
+            for (int d = kMaxDigits - 1; d >= 0; --d)
+                if (d < nd) cp = ldexp(D[d], p.cc.dig_exp[d]) + cp;
+            const int sa = (i < p.m) ? p.sig_a[i] : 0;
+            prods[t * (kCrtRows + 1) + r] = (sa == kNonFinite || sb == kNonFinite)
+                                                ? __longlong_as_double(0x7ff8000000000000ll)
+                                                : ldexp(cp, -(sa + sb));
         }
     }
-    const int sb = p.sig_b[j];
+    __syncthreads();
 #pragma unroll
     for (int r = 0; r < kCrtRows; ++r) {
-        const int i = i0 + r;
-        if (i >= p.m) break;
-        double xq = 0.0;
-#pragma unroll
-        for (int d = 0; d < kMaxDigits; ++d)
-            if (d < nd) xq = fma(S[r][d], p.cc.inv_p_dig[d], xq);
-        const double q = rint(xq);
-        // C' = sum_d D_d 2^(40 (nd-1-d)) with D_d = S_d - q P_d (exact, |D_d| < 2^53).
-        // Normalize to balanced 40-bit digits with exact carries, so that every
-        // lower part is below half an ulp of what is added on top of it; the sum
-        // is then within one rounding of C' and exact when C' is representable.
-        double D[kMaxDigits];
-#pragma unroll
-        for (int d = 0; d < kMaxDigits; ++d) D[d] = (d < nd) ? fma(-q, p.cc.p[d], S[r][d]) : 0.0;
-#pragma unroll
-        for (int d = kMaxDigits - 1; d >= 1; --d) {
-            if (d < nd) {
-                const double carry = rint(D[d] * 0x1p-40);
-                D[d] = fma(-carry, 0x1p40, D[d]);  // exact, |D_d| <= 2^39
-                D[d - 1] += carry;                 // exact, integers well below 2^53
-            }
+        const int e = t + kCrtThreads * r;  // row base + e of the block
+        const int i = base + e;
+        if (i < p.m) {
+            double* cptr = p.c + (int64_t)j * p.ldc + i;
+            *cptr = epilogue<double>(p.alpha, p.beta, prods[(e / kCrtRows) * (kCrtRows + 1) + e % kCrtRows], cptr);
         }
-        double cp = 0.0;
-#pragma unroll
-        for (int d = kMaxDigits - 1; d >= 0; --d)
-            if (d < nd) cp = ldexp(D[d], p.cc.dig_exp[d]) + cp;
-        const int sa = p.sig_a[i];
-        const double prod = (sa == kNonFinite || sb == kNonFinite) ? __longlong_as_double(0x7ff8000000000000ll)
-                                                                   : ldexp(cp, -(sa + sb));
-        double* cptr = p.c + (int64_t)j * p.ldc + i;
-        *cptr = epilogue<double>(p.alpha, p.beta, prod, cptr);
     }
 }
 
@@ -1051,7 +1071,7 @@ static hipError_t i8_launch_cfg(I8Args p, int nout, hipStream_t s) {
     p.tiles_m = (p.m + BM - 1) / BM;
     p.tiles_n = (p.n + BN - 1) / BN;
     if (p.kp % BK != 0) return hipErrorInvalidValue;
-    const size_t lds = 2 * (size_t)(BM + BN) * (BK + 16);
+    const size_t lds = 2 * (size_t)(BM + BN) * (BK + ZG_I8_LDS_PAD);
     auto kern = i8_gemm_kernel<BM, BN, BK, WM, WN, OCC, MF>;
     hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
                                        (int)lds);
@@ -1197,8 +1217,8 @@ int ozaki_dgemm(bool ta, bool tb, int m, int n, int k, double alpha, const doubl
             ca.m = m; ca.n = n; ca.u = u; ca.u_plane = ia.u_plane; ca.ldu = ldu;
             ca.sig_a = sig_a; ca.sig_b = sig_b; ca.alpha = alpha; ca.beta = beta; ca.c = c; ca.ldc = ldc;
             ca.cc = cc;
-            dim3 grid((m + 256 * kCrtRows - 1) / (256 * kCrtRows), n);
-            crt_reconstruct_kernel<<<grid, 256, 0, s>>>(ca);
+            dim3 grid((m + kCrtThreads * kCrtRows - 1) / (kCrtThreads * kCrtRows), n);
+            crt_reconstruct_kernel<<<grid, kCrtThreads, 0, s>>>(ca);
             e = hipGetLastError();
         }
     } else if (e == hipSuccess) {
