@@ -627,15 +627,24 @@ void i8_gemm_kernel(I8Args p) {
         const int8_t* lb = stage_b(cur);
 #pragma unroll
         for (int kk = 0; kk < BK; kk += MF::KD) {
-            v4i af[FM], bf[FN];
-#pragma unroll
-            for (int i = 0; i < FM; ++i) af[i] = *reinterpret_cast<const v4i*>(&la[(wm0 + i * MS + lr) * RS + kk + lk]);
+            // B fragments of this k step stay in registers; A fragments are
+            // streamed one block row at a time, the next one read before the
+            // MFMAs of the current one (the scheduling barriers keep that
+            // order, so the LDS latency hides behind FN MFMAs)
+            v4i bf[FN];
 #pragma unroll
             for (int j = 0; j < FN; ++j) bf[j] = *reinterpret_cast<const v4i*>(&lb[(wn0 + j * MS + lr) * RS + kk + lk]);
+            v4i a_cur = *reinterpret_cast<const v4i*>(&la[(wm0 + lr) * RS + kk + lk]);
 #pragma unroll
-            for (int i = 0; i < FM; ++i)
+            for (int i = 0; i < FM; ++i) {
+                v4i a_next = a_cur;
+                if (i + 1 < FM) a_next = *reinterpret_cast<const v4i*>(&la[(wm0 + (i + 1) * MS + lr) * RS + kk + lk]);
+                __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-                for (int j = 0; j < FN; ++j) acc[i][j] = MF::mma(af[i], bf[j], acc[i][j]);
+                for (int j = 0; j < FN; ++j) acc[i][j] = MF::mma(a_cur, bf[j], acc[i][j]);
+                __builtin_amdgcn_sched_barrier(0);
+                a_cur = a_next;
+            }
         }
         if (more) store(cur ^ 1);
         if constexpr (FL) {
