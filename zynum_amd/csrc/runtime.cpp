@@ -3,6 +3,7 @@
 // sequence of kernel launches on the current HIP device.
 #include "runtime.hpp"
 
+#include <pthread.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -116,8 +117,9 @@ class CopyPool {
     int threads() const { return nthreads_; }
 
     // Run fn(0) .. fn(parts - 1), spread over the workers and the caller.
+    // A forked child has none of the parent's workers: it copies alone.
     void run(int parts, const std::function<void(int)>& fn) {
-        if (parts <= 1 || nthreads_ <= 1) {
+        if (parts <= 1 || nthreads_ <= 1 || forked_.load()) {
             for (int i = 0; i < parts; ++i) fn(i);
             return;
         }
@@ -144,6 +146,7 @@ class CopyPool {
         if (hw > 0) n = std::min(n, (int)std::max(1u, hw / 2));
         if (const char* env = getenv("ZYNUM_GPU_COPY_THREADS")) n = std::max(1, std::min(64, atoi(env)));
         nthreads_ = n;
+        pthread_atfork(nullptr, nullptr, [] { CopyPool::get().forked_.store(true); });
         for (int i = 1; i < n; ++i) {
             workers_.emplace_back([this] { loop(); });
             workers_.back().detach();
@@ -169,6 +172,7 @@ class CopyPool {
     }
 
     int nthreads_ = 1;
+    std::atomic<bool> forked_{false};
     std::vector<std::thread> workers_;
     std::mutex run_mu_, mu_;
     std::condition_variable cv_, done_cv_;
