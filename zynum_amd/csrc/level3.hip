@@ -67,6 +67,17 @@ __device__ __forceinline__ T op_a(const LeafArgs<T>& p, int r, int c) {
 // unknown), so the per-step shuffle and FMA latencies of one vector overlap
 // with the others'.
 constexpr int kLeafVec = 16;
+
+// Value of v in lane `lane` (wave-uniform) of this wavefront.
+__device__ __forceinline__ double read_lane(double v, int lane) {
+    const uint64_t u = __double_as_longlong(v);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)u, lane);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(u >> 32), lane);
+    return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+__device__ __forceinline__ float read_lane(float v, int lane) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
 constexpr int kChains = kLeafVec / 4;
 
 template <typename T>
@@ -80,53 +91,79 @@ __global__ __launch_bounds__(256) void trsm_leaf_kernel(LeafArgs<T> p) {
     // side R solves x op(A) = b, i.e. op(A)^T x^T = b^T (M = op(A)^T block).
     // M is lower iff (side L and op(A) lower) or (side R and op(A) upper).
     const bool m_lower = p.left ? p.lower_eff : !p.lower_eff;
-    for (int e = tid; e < kLeaf * kLeaf; e += 256) {
+    // All loads of a thread are issued before any is used: indices are
+    // clamped into the block (always in bounds) and the unused values are
+    // replaced afterwards, so no load waits behind another one's branch.
+    constexpr int kMLoads = kLeaf * kLeaf / 256, kVLoads = kLeafVec * kLeaf / 256;
+    T mv[kMLoads];
+#pragma unroll
+    for (int t = 0; t < kMLoads; ++t) {
+        const int e = tid + 256 * t;
+        const int i = min(e % kLeaf, nb - 1), k = min(e / kLeaf, nb - 1);
+        mv[t] = p.left ? op_a(p, p.i0 + i, p.i0 + k) : op_a(p, p.i0 + k, p.i0 + i);
+    }
+    T bv[kVLoads];
+#pragma unroll
+    for (int t = 0; t < kVLoads; ++t) {
+        const int e = tid + 256 * t;
+        int v, i;
+        if (p.left) { i = e % kLeaf; v = e / kLeaf; } else { v = e % kLeafVec; i = e / kLeafVec; }
+        i = min(i, nb - 1);
+        v = min(v0 + v, p.nvec - 1);
+        bv[t] = p.left ? p.b[(int64_t)v * p.ldb + p.i0 + i] : p.b[(int64_t)(p.i0 + i) * p.ldb + v];
+    }
+#pragma unroll
+    for (int t = 0; t < kMLoads; ++t) {
+        const int e = tid + 256 * t;
         const int i = e % kLeaf, k = e / kLeaf;
         T v = T(0);
         if (i < nb && k < nb) {
-            if (i == k) {
-                v = p.unit ? T(1) : (p.left ? op_a(p, p.i0 + i, p.i0 + k) : op_a(p, p.i0 + k, p.i0 + i));
-            } else if (m_lower ? (i > k) : (i < k)) {
-                v = p.left ? op_a(p, p.i0 + i, p.i0 + k) : op_a(p, p.i0 + k, p.i0 + i);
-            }
+            if (i == k) v = p.unit ? T(1) : mv[t];
+            else if (m_lower ? (i > k) : (i < k)) v = mv[t];
         }
         M[i][k] = v;
     }
     // vectors: side L, vector v = column v0 + v of B, unknown i = row i0 + i;
     // side R, vector v = row v0 + v, unknown i = column i0 + i
-    for (int e = tid; e < kLeafVec * kLeaf; e += 256) {
+#pragma unroll
+    for (int t = 0; t < kVLoads; ++t) {
+        const int e = tid + 256 * t;
         int v, i;
         if (p.left) { i = e % kLeaf; v = e / kLeaf; } else { v = e % kLeafVec; i = e / kLeafVec; }
-        T x = T(0);
-        if (i < nb && v0 + v < p.nvec)
-            x = p.left ? p.b[(int64_t)(v0 + v) * p.ldb + p.i0 + i] : p.b[(int64_t)(p.i0 + i) * p.ldb + v0 + v];
-        V[v][i] = x;
+        V[v][i] = (i < nb && v0 + v < p.nvec) ? bv[t] : T(0);
     }
     __syncthreads();
     T x[kChains];
 #pragma unroll
     for (int c = 0; c < kChains; ++c) x[c] = V[wave * kChains + c][lane];
-    if (m_lower) {
-        for (int k = 0; k < nb; ++k) {
-            const T mk = M[lane][k];
-            const T dk = M[k][k];
+    // Step k: unknown k of every chain is read from lane k into scalar
+    // registers (k is wave-uniform), divided by the diagonal (lane c of one
+    // vector division holds chain c's numerator, so all chains share one
+    // correctly rounded division), kept by lane k and eliminated from the
+    // equations of the other lanes.  The chains are updated with selects, not
+    // branches.
+    const bool unit = p.unit;
+    for (int step = 0; step < nb; ++step) {
+        const int k = m_lower ? step : nb - 1 - step;
+        const T mk = M[lane][k];
+        const T dk = M[k][k];
+        const bool below = m_lower ? (lane > k && lane < nb) : (lane < k);
+        T xk[kChains];
 #pragma unroll
-            for (int c = 0; c < kChains; ++c) {
-                if (lane == k && !p.unit) x[c] = x[c] / dk;
-                const T xk = __shfl(x[c], k);
-                if (lane > k && lane < nb) x[c] = fma_t(mk, -xk, x[c]);
-            }
+        for (int c = 0; c < kChains; ++c) xk[c] = read_lane(x[c], k);
+        if (!unit) {
+            // one division for all chains: lane c divides chain c's value
+            T num = xk[0];
+#pragma unroll
+            for (int c = 1; c < kChains; ++c) num = (lane == c) ? xk[c] : num;
+            const T quo = num / dk;
+#pragma unroll
+            for (int c = 0; c < kChains; ++c) xk[c] = read_lane(quo, c);
         }
-    } else {
-        for (int k = nb - 1; k >= 0; --k) {
-            const T mk = M[lane][k];
-            const T dk = M[k][k];
 #pragma unroll
-            for (int c = 0; c < kChains; ++c) {
-                if (lane == k && !p.unit) x[c] = x[c] / dk;
-                const T xk = __shfl(x[c], k);
-                if (lane < k) x[c] = fma_t(mk, -xk, x[c]);
-            }
+        for (int c = 0; c < kChains; ++c) {
+            const T upd = fma_t(mk, -xk[c], x[c]);
+            x[c] = (lane == k) ? xk[c] : (below ? upd : x[c]);
         }
     }
 #pragma unroll
