@@ -297,13 +297,16 @@ int trsm_device(char side, char uplo, bool trans, char diag, int m, int n, T alp
 // ---------------------------------------------------------------------------
 // SYRK
 // ---------------------------------------------------------------------------
-using SyrkCfg = TileCfg<128, 128, 16, 4, 2, 2>;
+// The large-shape GEMM tile: 16 waves of 32x32, two workgroups per CU; BK 16
+// for FP64, 32 for FP32 (the same bytes per slice).
+template <typename T>
+using SyrkCfg = TileCfg<128, 128, (sizeof(T) == 8) ? 16 : 32, 4, 4, 4>;
 
-template <typename T, bool TA, int TRI>
-__global__ __launch_bounds__(SyrkCfg::NT) __attribute__((amdgpu_waves_per_eu(2)))
+template <typename T, bool TA, int TRI, int VEC>
+__global__ __launch_bounds__(SyrkCfg<T>::NT) __attribute__((amdgpu_waves_per_eu(SyrkCfg<T>::OCC)))
 void syrk_kernel(GemmArgs<T> p, int tiles) {
-    __shared__ __attribute__((aligned(16))) T lds_a[tile_lds_a<T, SyrkCfg>()];
-    __shared__ __attribute__((aligned(16))) T lds_b[tile_lds_b<T, SyrkCfg>()];
+    __shared__ __attribute__((aligned(16))) T lds_a[tile_lds_a<T, SyrkCfg<T>>()];
+    __shared__ __attribute__((aligned(16))) T lds_b[tile_lds_b<T, SyrkCfg<T>>()];
     // linear index -> (ti, tj) with ti >= tj (lower) or ti <= tj (upper)
     const int t = blockIdx.x;
     int hi = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
@@ -313,7 +316,25 @@ void syrk_kernel(GemmArgs<T> p, int tiles) {
     const int ti = (TRI == 1) ? hi : lo, tj = (TRI == 1) ? lo : hi;
     (void)tiles;
     // C = alpha * op(A) op(A)^T + beta * C: the B operand is A with the other transpose
-    gemm_tile<T, SyrkCfg, TA, !TA, 1, TRI>(p, ti * SyrkCfg::BM, tj * SyrkCfg::BN, lds_a, lds_b);
+    gemm_tile<T, SyrkCfg<T>, TA, !TA, VEC, TRI>(p, ti * SyrkCfg<T>::BM, tj * SyrkCfg<T>::BN, lds_a, lds_b);
+}
+
+// Triangle-tile launch: 16-byte operand loads when both operands allow them.
+template <typename T>
+hipError_t syrk_launch(const GemmArgs<T>& p, int tiles, bool lower, bool trans, hipStream_t s) {
+    constexpr int V = 16 / sizeof(T);
+    const bool vec = ((reinterpret_cast<uintptr_t>(p.a) & 15) == 0) && ((reinterpret_cast<uintptr_t>(p.b) & 15) == 0) &&
+                     (p.lda % V == 0) && (p.ldb % V == 0);
+#define ZG_SYRK(TA, TRI, VV) syrk_kernel<T, TA, TRI, VV><<<tiles, SyrkCfg<T>::NT, 0, s>>>(p, tiles)
+    if (vec) {
+        if (lower) { if (trans) ZG_SYRK(true, 1, V); else ZG_SYRK(false, 1, V); }
+        else { if (trans) ZG_SYRK(true, 2, V); else ZG_SYRK(false, 2, V); }
+    } else {
+        if (lower) { if (trans) ZG_SYRK(true, 1, 1); else ZG_SYRK(false, 1, 1); }
+        else { if (trans) ZG_SYRK(true, 2, 1); else ZG_SYRK(false, 2, 1); }
+    }
+#undef ZG_SYRK
+    return hipGetLastError();
 }
 
 // stored triangle of C <- W + beta * C (W = alpha * op(A) op(A)^T computed in
@@ -363,18 +384,11 @@ int syrk_device(char uplo, bool trans, int n, int k, T alpha, const T* a, int64_
     p.lda = lda; p.ldb = lda; p.ldc = ldc;
     p.alpha = alpha; p.beta = beta;
     p.a = a; p.b = a; p.c = c;
-    const int tn = (n + SyrkCfg::BM - 1) / SyrkCfg::BM;
+    const int tn = (n + SyrkCfg<T>::BM - 1) / SyrkCfg<T>::BM;
     p.tiles_m = p.tiles_n = tn;
     p.group_m = 1;
     const int tiles = tn * (tn + 1) / 2;
-    if (lower) {
-        if (trans) syrk_kernel<T, true, 1><<<tiles, SyrkCfg::NT, 0, s>>>(p, tiles);
-        else syrk_kernel<T, false, 1><<<tiles, SyrkCfg::NT, 0, s>>>(p, tiles);
-    } else {
-        if (trans) syrk_kernel<T, true, 2><<<tiles, SyrkCfg::NT, 0, s>>>(p, tiles);
-        else syrk_kernel<T, false, 2><<<tiles, SyrkCfg::NT, 0, s>>>(p, tiles);
-    }
-    return status_from_hip(hipGetLastError());
+    return status_from_hip(syrk_launch<T>(p, tiles, lower, trans, s));
 }
 
 // ---------------------------------------------------------------------------
@@ -545,7 +559,7 @@ int syr2k_device(char uplo, bool trans, int n, int k, T alpha, const T* a, int64
     p.ldc = ldc;
     p.alpha = alpha;
     p.c = c;
-    const int tn = (n + SyrkCfg::BM - 1) / SyrkCfg::BM;
+    const int tn = (n + SyrkCfg<T>::BM - 1) / SyrkCfg<T>::BM;
     p.tiles_m = p.tiles_n = tn;
     p.group_m = 1;
     const int tiles = tn * (tn + 1) / 2;
@@ -556,14 +570,7 @@ int syr2k_device(char uplo, bool trans, int n, int k, T alpha, const T* a, int64
         p.b = pass ? a : b;
         p.ldb = pass ? lda : ldb;
         p.beta = pass ? T(1) : beta;
-        if (lower) {
-            if (trans) syrk_kernel<T, true, 1><<<tiles, SyrkCfg::NT, 0, s>>>(p, tiles);
-            else syrk_kernel<T, false, 1><<<tiles, SyrkCfg::NT, 0, s>>>(p, tiles);
-        } else {
-            if (trans) syrk_kernel<T, true, 2><<<tiles, SyrkCfg::NT, 0, s>>>(p, tiles);
-            else syrk_kernel<T, false, 2><<<tiles, SyrkCfg::NT, 0, s>>>(p, tiles);
-        }
-        e = hipGetLastError();
+        e = syrk_launch<T>(p, tiles, lower, trans, s);
     }
     return status_from_hip(e);
 }
