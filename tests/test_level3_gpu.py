@@ -208,3 +208,46 @@ def test_dsyrk_random_vs_oracle(torch_cuda, oracle):
     G, R = got.reshape(n, n).T, ref.reshape(n, n).T
     tri = np.tril_indices(n)
     assert np.all(np.abs(G[tri] - R[tri]) <= bound[tri])
+
+
+@pytest.mark.parametrize("uplo", ["L", "U"])
+@pytest.mark.parametrize("trans", ["N", "T"])
+@pytest.mark.parametrize("n,k", [(301, 1100), (4096, 1024)])
+def test_dsyrk_split_tail_integer_exact(torch_cuda, uplo, trans, n, k):
+    """K >= 512 with a short last round of triangle tiles (all of them for
+    n = 301; 16 of 528 for n = 4096): those tiles are computed as K ranges and
+    summed before the epilogue.  Integer inputs: exact, the other triangle
+    and the padding untouched."""
+    rng = np.random.default_rng(n + k)
+    ar, ac = (n, k) if trans == "N" else (k, n)
+    lda, ldc = ar + 2, n + 1
+    A = rng.integers(-9, 10, (ar, ac)).astype(np.float64)
+    aa = np.full((ac, lda), -555.0)
+    aa[:, :ar] = A.T
+    Cm = rng.integers(-9, 10, (n, n)).astype(np.float64)
+    cc = np.full((n, ldc), -777.0)
+    cc[:, :n] = Cm.T
+    got = syrk(torch_cuda, uplo, trans, n, k, -0.5, aa.ravel(), lda, 2.0, cc.ravel(), ldc)
+    G = got.reshape(n, ldc)
+    X = A if trans == "N" else A.T
+    want = -0.5 * (X @ X.T) + 2.0 * Cm  # every partial sum is an integer below 2^53
+    GT = G[:, :n].T
+    lower = np.tril(np.ones((n, n), dtype=bool))
+    tri = lower if uplo == "L" else lower.T
+    assert np.array_equal(GT[tri], want[tri])
+    assert np.array_equal(GT[~tri], Cm[~tri])
+    assert np.all(G[:, n:] == -777.0)
+
+
+def test_dsyrk_split_tail_random_within_bound(torch_cuda, oracle):
+    rng = np.random.default_rng(21)
+    n, k = 700, 2000
+    a = rng.uniform(-1, 1, n * k)
+    c = rng.uniform(-1, 1, n * n)
+    got = syrk(torch_cuda, "U", "N", n, k, 1.5, a, n, 0.5, c, n)
+    A = a.reshape(k, n).T
+    exact = 1.5 * (A @ A.T) + 0.5 * c.reshape(n, n).T
+    bound = 2 * (k + 2) * 2.0**-53 * (1.5 * (np.abs(A) @ np.abs(A).T) + 0.5 * np.abs(c.reshape(n, n).T))
+    G = got.reshape(n, n).T
+    tri = np.triu_indices(n)
+    assert np.all(np.abs(G[tri] - exact[tri]) <= 2 * bound[tri])

@@ -153,7 +153,7 @@ def test_trmm_integer_exact(torch_cuda, oracle, single, side, uplo, trans, diag,
 @pytest.mark.parametrize("single", [False, True])
 @pytest.mark.parametrize("uplo", "UL")
 @pytest.mark.parametrize("trans", "NT")
-@pytest.mark.parametrize("shape", [(8, 6), (300, 77), (129, 260)])
+@pytest.mark.parametrize("shape", [(8, 6), (300, 77), (129, 260), (300, 1100)])
 def test_syr2k_integer_exact(torch_cuda, oracle, single, uplo, trans, shape):
     n, k = shape
     rows, cols = (n, k) if trans == "N" else (k, n)

@@ -302,21 +302,53 @@ int trsm_device(char side, char uplo, bool trans, char diag, int m, int n, T alp
 template <typename T>
 using SyrkCfg = TileCfg<128, 128, (sizeof(T) == 8) ? 16 : 32, 4, 4, 4>;
 
+// Triangle tiles in a cache-friendly order: the lower triangle of the tn x tn
+// tile grid is cut into 8 x 8 super-tiles, walked in triangular order, with
+// the tiles of each super-tile in row order; the workgroup index is first
+// remapped so each XCD takes a contiguous run of that order.  The workgroups
+// running together on one XCD then share a few A and B panels, as the grouped
+// order of the plain GEMM does.  TRI 2 (upper) takes the transposed tile.
+constexpr int kSyrkGroup = 8;
+
+template <int TRI>
+__device__ __forceinline__ void tri_tile(int t, int tn, int& ti, int& tj) {
+    const int G = kSyrkGroup, ns = (tn + G - 1) / G;
+    int I = 0, J = 0;
+    for (I = 0; I < ns; ++I) {
+        const int r0 = I * G, r1 = min(tn, r0 + G);
+        bool found = false;
+        for (J = 0; J <= I; ++J) {
+            const int c0 = J * G, c1 = min(tn, c0 + G);
+            int cnt = 0;  // tiles (i, j) of the super-tile with j <= i
+            if (J < I) cnt = (r1 - r0) * (c1 - c0);
+            else for (int i = r0; i < r1; ++i) cnt += min(c1, i + 1) - c0;
+            if (t < cnt) { found = true; break; }
+            t -= cnt;
+        }
+        if (found) break;
+    }
+    const int r0 = I * G, c0 = J * G, c1 = min(tn, c0 + G);
+    int i = r0;
+    for (;; ++i) {
+        const int w = (J < I) ? (c1 - c0) : (min(c1, i + 1) - c0);
+        if (t < w) break;
+        t -= w;
+    }
+    const int j = c0 + t;
+    ti = (TRI == 1) ? i : j;
+    tj = (TRI == 1) ? j : i;
+}
+
 template <typename T, bool TA, int TRI, int VEC>
 __global__ __launch_bounds__(SyrkCfg<T>::NT) __attribute__((amdgpu_waves_per_eu(SyrkCfg<T>::OCC)))
 void syrk_kernel(GemmArgs<T> p, int tiles) {
-    __shared__ __attribute__((aligned(16))) T lds_a[tile_lds_a<T, SyrkCfg<T>>()];
-    __shared__ __attribute__((aligned(16))) T lds_b[tile_lds_b<T, SyrkCfg<T>>()];
-    // linear index -> (ti, tj) with ti >= tj (lower) or ti <= tj (upper)
-    const int t = blockIdx.x;
-    int hi = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
-    while ((hi + 1) * (hi + 2) / 2 <= t) ++hi;
-    while (hi * (hi + 1) / 2 > t) --hi;
-    const int lo = t - hi * (hi + 1) / 2;
-    const int ti = (TRI == 1) ? hi : lo, tj = (TRI == 1) ? lo : hi;
-    (void)tiles;
+    using CF = SyrkCfg<T>;
+    __shared__ __attribute__((aligned(16))) T lds_a[tile_lds_a<T, CF>()];
+    __shared__ __attribute__((aligned(16))) T lds_b[tile_lds_b<T, CF>()];
+    int ti, tj;
+    tri_tile<TRI>(xcd_remap(blockIdx.x, tiles), p.tiles_m, ti, tj);
     // C = alpha * op(A) op(A)^T + beta * C: the B operand is A with the other transpose
-    gemm_tile<T, SyrkCfg<T>, TA, !TA, VEC, TRI>(p, ti * SyrkCfg<T>::BM, tj * SyrkCfg<T>::BN, lds_a, lds_b);
+    gemm_tile<T, CF, TA, !TA, VEC, TRI>(p, ti * CF::BM, tj * CF::BN, lds_a, lds_b);
 }
 
 // Triangle-tile launch: 16-byte operand loads when both operands allow them.
