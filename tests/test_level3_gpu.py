@@ -213,11 +213,10 @@ def test_dsyrk_random_vs_oracle(torch_cuda, oracle):
 @pytest.mark.parametrize("uplo", ["L", "U"])
 @pytest.mark.parametrize("trans", ["N", "T"])
 @pytest.mark.parametrize("n,k", [(301, 1100), (4096, 1024)])
-def test_dsyrk_split_tail_integer_exact(torch_cuda, uplo, trans, n, k):
-    """K >= 512 with a short last round of triangle tiles (all of them for
-    n = 301; 16 of 528 for n = 4096): those tiles are computed as K ranges and
-    summed before the epilogue.  Integer inputs: exact, the other triangle
-    and the padding untouched."""
+def test_dsyrk_long_k_integer_exact(torch_cuda, uplo, trans, n, k):
+    """Long K over few and many triangle tiles (6 for n = 301; 528 for
+    n = 4096, walked in 8 x 8 super-tiles with a partial last super-row).
+    Integer inputs: exact, the other triangle and the padding untouched."""
     rng = np.random.default_rng(n + k)
     ar, ac = (n, k) if trans == "N" else (k, n)
     lda, ldc = ar + 2, n + 1
@@ -239,7 +238,7 @@ def test_dsyrk_split_tail_integer_exact(torch_cuda, uplo, trans, n, k):
     assert np.all(G[:, n:] == -777.0)
 
 
-def test_dsyrk_split_tail_random_within_bound(torch_cuda, oracle):
+def test_dsyrk_long_k_random_within_bound(torch_cuda, oracle):
     rng = np.random.default_rng(21)
     n, k = 700, 2000
     a = rng.uniform(-1, 1, n * k)
