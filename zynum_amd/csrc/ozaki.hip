@@ -1194,16 +1194,26 @@ int ozaki_dgemm(bool ta, bool tb, int m, int n, int k, double alpha, const doubl
     int8_t* pb = ws.take<int8_t>(plane_b * planes_n);
 
     hipError_t e = hipSuccess;
+    // The two operands' statistics and planes are independent: B's run on the
+    // auxiliary stream beside A's (both memory-bound, neither fills HBM alone)
+    // and are joined back before the products.
+    hipStream_t side = nullptr;
+    const bool forked = aux_fork(s, &side);
+    hipStream_t sb = forked ? side : s;
     // op(A) rows are contiguous in k when transposed; op(B) "rows" (its columns) when not
     e = operand_stats(a, lda, m, k, ta, max_a, ss_a, s);
-    if (e == hipSuccess) e = operand_stats(b, ldb, n, k, !tb, max_b, ss_b, s);
+    if (e == hipSuccess) e = operand_stats(b, ldb, n, k, !tb, max_b, ss_b, sb);
     if (e == hipSuccess) {
         row_exponent_kernel<<<(m + 255) / 256, 256, 0, s>>>(m, k, max_a, ss_a, nchunks, cc.alpha, crt, sig_a);
-        row_exponent_kernel<<<(n + 255) / 256, 256, 0, s>>>(n, k, max_b, ss_b, nchunks, cc.alpha, crt, sig_b);
+        row_exponent_kernel<<<(n + 255) / 256, 256, 0, sb>>>(n, k, max_b, ss_b, nchunks, cc.alpha, crt, sig_b);
         e = hipGetLastError();
     }
     if (e == hipSuccess) e = planes(a, lda, m, k, kp, ta, sig_a, pa, plane_a, planes_n, crt, &cc, s, bf16);
-    if (e == hipSuccess) e = planes(b, ldb, n, k, kp, !tb, sig_b, pb, plane_b, planes_n, crt, &cc, s, bf16);
+    if (e == hipSuccess) e = planes(b, ldb, n, k, kp, !tb, sig_b, pb, plane_b, planes_n, crt, &cc, sb, bf16);
+    if (forked) {
+        const hipError_t ej = aux_join(s, side);  // always joined: the workspace is released after s drains
+        if (e == hipSuccess) e = ej;
+    }
 
     I8Args ia;
     memset(&ia, 0, sizeof(ia));
@@ -1256,7 +1266,9 @@ int ozaki_dgemm(bool ta, bool tb, int m, int n, int k, double alpha, const doubl
         }
     }
     int st = status_from_hip(e);
-    if (st == ZYNUM_GPU_OK) st = status_from_hip(hipStreamSynchronize(s));  // workspace reuse is host-ordered
+    const hipError_t es = hipStreamSynchronize(s);  // workspace reuse is host-ordered
+    if (st == ZYNUM_GPU_OK) st = status_from_hip(es);
+    if (forked && st != ZYNUM_GPU_OK) (void)hipStreamSynchronize(side);
     workspace_release();
     return st;
 }

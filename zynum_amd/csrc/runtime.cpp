@@ -369,6 +369,60 @@ void Operand::release() {
     }
 }
 
+namespace {
+struct AuxStream {
+    hipStream_t stream = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
+};
+constexpr int kAuxDevices = 64;
+AuxStream g_aux[kAuxDevices];
+
+AuxStream* aux_for_current_device() {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kAuxDevices) {
+        (void)hipGetLastError();
+        return nullptr;
+    }
+    AuxStream& a = g_aux[dev];
+    if (a.stream == nullptr) {
+        hipStream_t st = nullptr;
+        hipEvent_t f = nullptr, j = nullptr;
+        if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess ||
+            hipEventCreateWithFlags(&f, hipEventDisableTiming) != hipSuccess ||
+            hipEventCreateWithFlags(&j, hipEventDisableTiming) != hipSuccess) {
+            (void)hipGetLastError();
+            if (j) (void)hipEventDestroy(j);
+            if (f) (void)hipEventDestroy(f);
+            if (st) (void)hipStreamDestroy(st);
+            return nullptr;
+        }
+        a.fork = f;
+        a.join = j;
+        a.stream = st;
+    }
+    return &a;
+}
+}  // namespace
+
+bool aux_fork(hipStream_t s, hipStream_t* side) {
+    AuxStream* a = aux_for_current_device();
+    if (a == nullptr) return false;
+    if (hipEventRecord(a->fork, s) != hipSuccess || hipStreamWaitEvent(a->stream, a->fork, 0) != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    *side = a->stream;
+    return true;
+}
+
+hipError_t aux_join(hipStream_t s, hipStream_t side) {
+    AuxStream* a = aux_for_current_device();
+    if (a == nullptr || a->stream != side) return hipErrorInvalidValue;
+    hipError_t e = hipEventRecord(a->join, side);
+    if (e == hipSuccess) e = hipStreamWaitEvent(s, a->join, 0);
+    return e;
+}
+
 void* scratch_acquire(size_t bytes, size_t* got) { return g_stage_dev.acquire(bytes, got); }
 
 void scratch_release(void* p, size_t bytes) { g_stage_dev.release(p, bytes); }

@@ -69,6 +69,15 @@ void workspace_free_all();
 void* scratch_acquire(size_t bytes, size_t* got);  // nullptr on allocation failure
 void scratch_release(void* p, size_t bytes);
 
+// A second stream (non-blocking, default priority) per device for work of a
+// call that is independent of what its stream is doing: aux_fork makes it
+// wait for everything queued so far on `s`; aux_join makes `s` wait for
+// everything queued on it.  The fork/join events are shared, so callers hold
+// the workspace lock between the two.  aux_fork returns false (nothing forked)
+// when the stream or events cannot be created.
+bool aux_fork(hipStream_t s, hipStream_t* side);
+hipError_t aux_join(hipStream_t s, hipStream_t side);
+
 int status_from_hip(hipError_t e);
 
 }  // namespace zg
