@@ -283,9 +283,18 @@ __device__ __forceinline__ int sym_mod(double v, int m, double inv_m) {
     return ri;
 }
 
+// Rows of op(X) per block of the plane kernel; each of the 256 threads then
+// converts EPT = TR * 128 / 256 consecutive k of one row.
+#ifndef ZG_PLANE_ROWS
+#define ZG_PLANE_ROWS 16
+#endif
+
 template <bool KCONTIG>
 __global__ __launch_bounds__(256) void plane_kernel(PlaneArgs p) {
-    constexpr int TR = 32, TK = 128;
+    constexpr int TR = ZG_PLANE_ROWS, TK = 128;
+    constexpr int EPT = TR * TK / 256;  // elements per thread (8 or 16)
+    constexpr int TPR = TK / EPT;       // threads per row
+    static_assert((EPT == 8 || EPT == 16) && TPR * EPT == TK, "plane tile: 16 or 32 rows per block");
     __shared__ double tile[TR][TK + 1];
     const int r0 = blockIdx.x * TR, k0 = blockIdx.y * TK;
     const int t = threadIdx.x;
@@ -302,31 +311,32 @@ __global__ __launch_bounds__(256) void plane_kernel(PlaneArgs p) {
         tile[r][kk] = v;
     }
     __syncthreads();
-    // each thread: one row, 16 consecutive k -> 16 (bf16: 32) bytes per plane
-    const int r = t >> 3, kq = (t & 7) * 16;
+    // each thread: one row, EPT consecutive k -> EPT (bf16: 2 EPT) bytes per plane
+    const int r = t / TPR, kq = (t % TPR) * EPT;
     const int gr = r0 + r;
     if (gr >= p.rows || k0 + kq >= p.kp) return;
     const int sg = p.sigma[gr];
     const int eb = p.bf16 ? 2 : 1;
     int8_t* dst = p.out + ((int64_t)gr * p.kp + k0 + kq) * eb;
+    constexpr int W = EPT / 4;  // 32-bit words of int8 output per plane
     if (sg == kNonFinite) {
         // the row's results are set to NaN by the final kernel; its planes are zero
-        for (int l = 0; l < p.nplanes; ++l)
-            for (int h = 0; h < eb; ++h)
-                reinterpret_cast<int4*>(dst + (int64_t)l * p.plane_stride)[h] = int4{0, 0, 0, 0};
+        for (int l = 0; l < p.nplanes; ++l) {
+            uint32_t* d = reinterpret_cast<uint32_t*>(dst + (int64_t)l * p.plane_stride);
+            for (int w = 0; w < W * eb; ++w) d[w] = 0u;
+        }
         return;
     }
     if (p.crt) {
-        double a[16];
+        double a[EPT];
 #pragma unroll
-        for (int b = 0; b < 16; ++b) a[b] = rint(ldexp(tile[r][kq + b], sg));  // exact integers, |a| <= 2^(alpha+1)
+        for (int b = 0; b < EPT; ++b) a[b] = rint(ldexp(tile[r][kq + b], sg));  // exact integers, |a| <= 2^(alpha+1)
         for (int l = 0; l < p.nplanes; ++l) {
             const int ml = p.m[l];
             const double iml = p.inv_m[l];
-            int4 packed;
-            uint32_t* pw = reinterpret_cast<uint32_t*>(&packed);
+            uint32_t pw[W];
 #pragma unroll
-            for (int w = 0; w < 4; ++w) {
+            for (int w = 0; w < W; ++w) {
                 uint32_t word = 0;
 #pragma unroll
                 for (int b = 0; b < 4; ++b) {
@@ -335,42 +345,40 @@ __global__ __launch_bounds__(256) void plane_kernel(PlaneArgs p) {
                 }
                 pw[w] = word;
             }
-            *reinterpret_cast<int4*>(dst + (int64_t)l * p.plane_stride) = packed;
+            uint32_t* d = reinterpret_cast<uint32_t*>(dst + (int64_t)l * p.plane_stride);
+            if constexpr (W == 4) *reinterpret_cast<uint4*>(d) = uint4{pw[0], pw[1], pw[2], pw[3]};
+            else *reinterpret_cast<uint2*>(d) = uint2{pw[0], pw[1]};
         }
     } else {
         // slices: s_1 = rint(x 2^6), then 7 more bits per slice (|s_j| <= 64)
-        double rem[16];
+        double rem[EPT];
 #pragma unroll
-        for (int b = 0; b < 16; ++b) rem[b] = ldexp(tile[r][kq + b], sg);  // |rem| < 1
+        for (int b = 0; b < EPT; ++b) rem[b] = ldexp(tile[r][kq + b], sg);  // |rem| < 1
         for (int l = 0; l < p.nplanes; ++l) {
             const int shift = (l == 0) ? 6 : 6 + 7 * l;
-            double v[16];
+            double v[EPT];
 #pragma unroll
-            for (int b = 0; b < 16; ++b) {
+            for (int b = 0; b < EPT; ++b) {
                 v[b] = rint(ldexp(rem[b], shift));
                 rem[b] = fma(-v[b], ldexp(1.0, -shift), rem[b]);  // exact
             }
-            int8_t* d = dst + (int64_t)l * p.plane_stride;
+            uint32_t* d = reinterpret_cast<uint32_t*>(dst + (int64_t)l * p.plane_stride);
             if (p.bf16) {
                 // |v| <= 64: the float has at most 7 significant bits, so its top 16 bits are the bf16
-                int4 packed[2];
-                uint32_t* pw = reinterpret_cast<uint32_t*>(packed);
 #pragma unroll
-                for (int w = 0; w < 8; ++w)
-                    pw[w] = (__float_as_uint((float)v[2 * w]) >> 16) | (__float_as_uint((float)v[2 * w + 1]) & 0xffff0000u);
-                reinterpret_cast<int4*>(d)[0] = packed[0];
-                reinterpret_cast<int4*>(d)[1] = packed[1];
+                for (int w = 0; w < EPT / 2; ++w)
+                    d[w] = (__float_as_uint((float)v[2 * w]) >> 16) | (__float_as_uint((float)v[2 * w + 1]) & 0xffff0000u);
             } else {
-                int4 packed;
-                uint32_t* pw = reinterpret_cast<uint32_t*>(&packed);
+                uint32_t pw[W];
 #pragma unroll
-                for (int w = 0; w < 4; ++w) {
+                for (int w = 0; w < W; ++w) {
                     uint32_t word = 0;
 #pragma unroll
                     for (int b = 0; b < 4; ++b) word |= ((uint32_t)(uint8_t)(int8_t)(int)v[4 * w + b]) << (8 * b);
                     pw[w] = word;
                 }
-                *reinterpret_cast<int4*>(d) = packed;
+                if constexpr (W == 4) *reinterpret_cast<uint4*>(d) = uint4{pw[0], pw[1], pw[2], pw[3]};
+                else *reinterpret_cast<uint2*>(d) = uint2{pw[0], pw[1]};
             }
         }
     }
@@ -1068,7 +1076,7 @@ hipError_t planes(const double* x, int64_t ld, int rows, int k, int kp, bool kco
     pa.bf16 = bf16 ? 1 : 0;
     if (crt)
         for (int l = 0; l < nplanes; ++l) { pa.m[l] = cc->m[l]; pa.inv_m[l] = cc->inv_m[l]; }
-    dim3 grid((rows + 31) / 32, kp / 128);
+    dim3 grid((rows + ZG_PLANE_ROWS - 1) / ZG_PLANE_ROWS, kp / 128);
     if (kcontig) plane_kernel<true><<<grid, 256, 0, s>>>(pa);
     else plane_kernel<false><<<grid, 256, 0, s>>>(pa);
     return hipGetLastError();
