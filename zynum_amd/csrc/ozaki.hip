@@ -942,12 +942,18 @@ struct CrtArgs {
     CrtConsts cc;
 };
 
-// One thread per 8 consecutive rows of one column: the residue bytes of all
-// moduli are loaded first (one 64-bit load per modulus, all in flight), then
+// One thread per kCrtRows consecutive rows of one column: the residue bytes of
+// all moduli are loaded first (one 32- or 64-bit load per modulus, all in
+// flight; 4 rows per thread, 68 VGPRs and 7 waves per SIMD, measured the same
+// as 8 rows with 126 VGPRs and 4 waves), then
 // summed into the CRT digits.  Rows m..ldu-1 of a residue plane are padding.
 // The products go through LDS so that the block's 2048 rows of C are read (for
 // beta) and written with consecutive lanes on consecutive rows.
-constexpr int kCrtRows = 8;
+#ifndef ZG_CRT_ROWS
+#define ZG_CRT_ROWS 8
+#endif
+constexpr int kCrtRows = ZG_CRT_ROWS;  // 4 or 8
+static_assert(kCrtRows == 4 || kCrtRows == 8, "rows per reconstruction thread");
 constexpr int kCrtThreads = 256;
 __global__ __launch_bounds__(kCrtThreads) void crt_reconstruct_kernel(CrtArgs p) {
     __shared__ double prods[kCrtThreads * (kCrtRows + 1)];  // row stride 9: conflict-free 8-byte writes
@@ -959,17 +965,26 @@ __global__ __launch_bounds__(kCrtThreads) void crt_reconstruct_kernel(CrtArgs p)
     if (i0 < p.m) {
         const int nd = p.cc.ndig, nm = p.cc.nmod;
         const uint8_t* u = p.u + (int64_t)j * p.ldu + i0;
-        uint2 w[kMaxModuli];
+        uint32_t w[kMaxModuli][kCrtRows / 4];
 #pragma unroll
-        for (int l = 0; l < kMaxModuli; ++l)
-            if (l < nm) w[l] = *reinterpret_cast<const uint2*>(u + (int64_t)l * p.u_plane);
+        for (int l = 0; l < kMaxModuli; ++l) {
+            if (l < nm) {
+                if constexpr (kCrtRows == 8) {
+                    const uint2 v = *reinterpret_cast<const uint2*>(u + (int64_t)l * p.u_plane);
+                    w[l][0] = v.x;
+                    w[l][kCrtRows / 4 - 1] = v.y;
+                } else {
+                    w[l][0] = *reinterpret_cast<const uint32_t*>(u + (int64_t)l * p.u_plane);
+                }
+            }
+        }
         double S[kCrtRows][kMaxDigits] = {};
 #pragma unroll
         for (int l = 0; l < kMaxModuli; ++l) {
             if (l < nm) {
 #pragma unroll
                 for (int r = 0; r < kCrtRows; ++r) {
-                    const uint32_t word = (r < 4) ? w[l].x : w[l].y;
+                    const uint32_t word = w[l][r >> 2];
                     const double ul = (double)((word >> (8 * (r & 3))) & 0xffu);
 #pragma unroll
                     for (int d = 0; d < kMaxDigits; ++d)
