@@ -49,12 +49,12 @@ struct alignas(16) TileItem {
 // The small shapes run four waves in a 2x2 grid, so each wave issues a whole
 // (BM/2) x (BN/2) block of MFMAs per K slice: enough work per barrier to cover
 // the next slice's load latency with the short K of grouped problems.  The
-// 128x128 shape runs eight waves (4x2), the plain GEMM's best layout.
+// 128x128 shape runs sixteen waves (4x4 of 32x32), the large plain GEMM's tile.
 // OCC: waves per SIMD the registers are sized for.
 using GShape0 = TileCfg<32, 64, 16, 2, 2, 4>;
 using GShape1 = TileCfg<64, 32, 16, 2, 2, 4>;
 using GShape2 = TileCfg<64, 64, 16, 2, 2, 4>;
-using GShape3 = TileCfg<128, 128, 16, 4, 2, 2>;
+using GShape3 = TileCfg<128, 128, 16, 4, 4, 4>;
 constexpr int kShapeRows[4] = {32, 64, 64, 128};
 constexpr int kShapeCols[4] = {64, 32, 64, 128};
 
