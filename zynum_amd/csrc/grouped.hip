@@ -200,7 +200,6 @@ const void* path_kernel(int path) {
     }
 }
 constexpr int kPathThreads[4] = {GShape0::NT, GShape1::NT, GShape2::NT, GShape3::NT};
-constexpr int kNumPaths = 32;  // plus the scale path
 
 // ---------------------------------------------------------------------------
 // tile choice per size bucket
