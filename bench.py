@@ -194,19 +194,33 @@ def main() -> int:
                     variants[f"dgemm_{name}"] = round(3 * flop / (s.elapsed_time(e) * 1e-3) / 1e12, 2)
             del C1
             A32, B32 = A.float(), B.float()
-            C32 = torch.zeros(ld * n, dtype=torch.float32, device=dev)
-            step(scalar=_lib.F32, a=A32, b=B32, c=C32)
-            s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-            s.record(stream)
-            for _ in range(5):
-                step(scalar=_lib.F32, a=A32, b=B32, c=C32)
-            e.record(stream)
-            torch.cuda.synchronize()
-            variants["sgemm_NN_beta0"] = round(5 * flop / (s.elapsed_time(e) * 1e-3) / 1e12, 2)
-            variants["sgemm_frac_of_157.3TF"] = round(variants["sgemm_NN_beta0"] / 157.3, 4)
+            C32 = torch.randn(ld * n, dtype=torch.float32, device=dev)
+            for ta, tb in ((False, False), (False, True), (True, False), (True, True)):
+                for beta in (0.0, 1.0):
+                    name = f"{'T' if ta else 'N'}{'T' if tb else 'N'}_beta{int(beta)}"
+                    step(scalar=_lib.F32, a=A32, b=B32, c=C32, ta=ta, tb=tb, beta=beta)
+                    s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                    s.record(stream)
+                    for _ in range(5):
+                        step(scalar=_lib.F32, a=A32, b=B32, c=C32, ta=ta, tb=tb, beta=beta)
+                    e.record(stream)
+                    torch.cuda.synchronize()
+                    variants[f"sgemm_{name}"] = round(5 * flop / (s.elapsed_time(e) * 1e-3) / 1e12, 2)
+            variants["sgemm_frac_of_157.3TF"] = [round(min(v for k, v in variants.items() if k.startswith("sgemm_"))
+                                                       / 157.3, 4),
+                                                 round(max(v for k, v in variants.items() if k.startswith("sgemm_"))
+                                                       / 157.3, 4)]
             del A32, B32, C32
             result["variants_tflops"] = variants
             log(f"variants: {variants}")
+
+        # ---------------- configuration 1: synchronous cblas_dgemm 1024^3 -----------------
+        if not args.no_variants:
+            try:
+                result["cblas_dgemm_1024"] = config1_leg(torch, lib)
+            except Exception as exc:
+                result["cblas_dgemm_1024"] = {"error": str(exc)}
+                log(f"cblas_dgemm 1024 leg failed: {exc}")
 
         # ---------------- split precision (INT8 / CRT) -----------------
         if not args.no_split:
@@ -257,6 +271,45 @@ def main() -> int:
 
     print(json.dumps(result), flush=True)
     return 0
+
+
+def config1_leg(torch, lib, reps=20):
+    """cblas_dgemm(ColMajor, NoTrans, NoTrans, 1024, 1024, 1024, 1.5, A, 1024,
+    B, 1024, 0.5, C, 1024) on device operands (A, B, C uniform [-1, 1) from
+    seed 0), timed by the host clock around each synchronous call: the wall
+    time a BLAS caller sees, launch and synchronisation included."""
+    from zynum_amd import synth
+
+    n = 1024
+    dev = torch.device("cuda:0")
+    a = torch.from_numpy(synth.matrix(0, 0, n, n, n)).to(dev)
+    b = torch.from_numpy(synth.matrix(0, n * n, n, n, n)).to(dev)
+    c0 = torch.from_numpy(synth.matrix(0, 2 * n * n, n, n, n)).to(dev)
+    c = c0.clone()
+    torch.cuda.synchronize()
+
+    def call():
+        lib.cblas_dgemm(102, 111, 111, n, n, n, 1.5, a.data_ptr(), n, b.data_ptr(), n, 0.5, c.data_ptr(), n)
+
+    call()
+    if lib.zynum_gpu_last_status() != 0:
+        raise RuntimeError("cblas_dgemm did not run on the GPU tier")
+    times = []
+    for _ in range(reps):
+        c.copy_(c0)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        call()
+        times.append(time.perf_counter() - t0)
+    times.sort()
+    flop = 2.0 * n ** 3
+    out = {"workload": "cblas_dgemm ColMajor NN M=N=K=1024 ld=1024 alpha=1.5 beta=0.5, device operands, "
+                       f"synchronous calls, host clock, {reps} calls after 1 warm-up",
+           "best_us": round(times[0] * 1e6, 1), "median_us": round(times[len(times) // 2] * 1e6, 1),
+           "best_gflops": round(flop / times[0] / 1e9, 1),
+           "median_gflops": round(flop / times[len(times) // 2] / 1e9, 1)}
+    log(f"cblas_dgemm 1024: {out}")
+    return out
 
 
 def split_leg(torch, lib, A, B, n, ld, stream, plain_tf):
@@ -406,17 +459,28 @@ def level3_leg(torch, lib, n, stream):
         torch.cuda.synchronize()
         _lib.check(st, "dtrsm")
         ms = s.elapsed_time(e)
-        # residual L X - B with the plain FP64 GEMM (column-major L is the transposed row-major tensor)
-        R = B.clone()
+        # residual L X - B in double-double on the device (the dd product is
+        # independent of the GEMM core), against the backward-error scale
+        # n u (|L||X|) formed with the plain GEMM on absolute values
+        R_hi = torch.empty_like(B)
+        R_lo = torch.empty_like(B)
+        st = lib.zynum_gpu_dd_gemm(b"N", b"N", n, n, n, 1.0, L.data_ptr(), n, X.data_ptr(), n, -1.0, B.data_ptr(), n,
+                                   R_hi.data_ptr(), R_lo.data_ptr(), n, sh)
+        _lib.check(st, "dd residual")
+        R = (R_hi + R_lo).abs()
+        del R_hi, R_lo
+        La, Xa = L.abs(), X.abs()
+        scale = torch.zeros_like(B)
         from zynum_amd import blas
-        st = blas.gemm_request(False, False, n, n, n, 1.0, L.data_ptr(), n, X.data_ptr(), n, -1.0, R.data_ptr(), n,
-                               _lib.F64, stream=sh)
-        _lib.check(st, "residual")
+        _lib.check(blas.gemm_request(False, False, n, n, n, 1.0, La.data_ptr(), n, Xa.data_ptr(), n, 0.0,
+                                     scale.data_ptr(), n, _lib.F64, stream=sh), "residual scale")
         torch.cuda.synchronize()
+        del La, Xa
         nrm = float(L.abs().sum(dim=1).max()) * float(X.abs().max())
         res[tag] = {"ms": round(ms, 3), "tflops": round(flop / (ms * 1e-3) / 1e12, 2),
-                    "residual_max": float(R.abs().max()), "residual_normwise": float(R.abs().max()) / nrm}
-        del X, R
+                    "residual_max_dd": float(R.max()), "residual_normwise": float(R.max()) / nrm,
+                    "residual_over_n_u_LX_max": float((R / (n * 2.0**-53 * scale)).max())}
+        del X, R, scale
     out["dtrsm"] = res
     del L, B
     # --- DSYRK ---
@@ -428,24 +492,34 @@ def level3_leg(torch, lib, n, stream):
                                ref_hi.data_ptr(), ref_lo.data_ptr(), n, sh)
     _lib.check(st, "dd syrk reference")
     flop = float(n) * n * k
-    res = {"workload": f"DSYRK Lower NoTrans N={n} K={k}, beta=0, lower-triangle tiles of the GEMM core"}
+    res = {"workload": f"DSYRK Lower NoTrans N={n} K={k}, beta 0 and 1, lower-triangle tiles of the GEMM core "
+                       "(best of 3 calls)"}
     low = torch.tril(torch.ones(n, n, dtype=torch.bool, device=dev)).T  # column-major lower in row-major view
-    for mode, param, tag in ((_lib.MODE_PLAIN, 0, "plain_fp64"), (_lib.MODE_OZAKI_CRT, 14, "crt14")):
+    for mode, param, tag, beta in ((_lib.MODE_PLAIN, 0, "plain_fp64", 0.0), (_lib.MODE_PLAIN, 0, "plain_fp64_beta1", 1.0),
+                                   (_lib.MODE_OZAKI_CRT, 14, "crt14", 0.0)):
         C = torch.zeros(n * n, dtype=torch.float64, device=dev)
-        st = lib.zynum_gpu_dsyrk(b"L", b"N", n, k, 1.0, A.data_ptr(), n, 0.0, C.data_ptr(), n, sh, mode, param, 0)
+        st = lib.zynum_gpu_dsyrk(b"L", b"N", n, k, 1.0, A.data_ptr(), n, beta, C.data_ptr(), n, sh, mode, param, 0)
         _lib.check(st, "dsyrk")
+        C.zero_()  # beta = 1 then adds the product to exact zeros: the result is the product
         torch.cuda.synchronize()
         s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         s.record(stream)
-        for _ in range(3):
-            st = lib.zynum_gpu_dsyrk(b"L", b"N", n, k, 1.0, A.data_ptr(), n, 0.0, C.data_ptr(), n, sh, mode, param, 0)
+        st = lib.zynum_gpu_dsyrk(b"L", b"N", n, k, 1.0, A.data_ptr(), n, beta, C.data_ptr(), n, sh, mode, param, 0)
         e.record(stream)
         torch.cuda.synchronize()
         _lib.check(st, "dsyrk")
-        ms = s.elapsed_time(e) / 3
+        ms = s.elapsed_time(e)
         err = ((C - ref_hi) - ref_lo).view(n, n)[low].abs().max()
+        for _ in range(2):  # two more timed calls (beta = 1 accumulates; only the first is checked)
+            s.record(stream)
+            st = lib.zynum_gpu_dsyrk(b"L", b"N", n, k, 1.0, A.data_ptr(), n, beta, C.data_ptr(), n, sh, mode, param,
+                                     0)
+            e.record(stream)
+            torch.cuda.synchronize()
+            _lib.check(st, "dsyrk")
+            ms = min(ms, s.elapsed_time(e))
         res[tag] = {"ms": round(ms, 3), "tflops": round(flop / (ms * 1e-3) / 1e12, 2),
-                    "max_abs_err_vs_dd": float(err)}
+                    "max_abs_err_vs_dd": float(err), "beta": beta}
         del C
     out["dsyrk"] = res
     del A, ref_hi, ref_lo
@@ -686,7 +760,7 @@ def offload_crossover(lib, off, cpu_by_edge):
         f"{agree}/{len(OFFLOAD_EDGES)} edges")
 
 
-def cpu_leg(n, ld, target_s, edges=()):
+def cpu_leg(n, ld, target_s, edges=(), with_configs=True):
     """The oracle (restated reference CPU path: packed-B 16x8 FMA tiles, column
     panels over min(cores, 64) threads) on a column block of the same workload,
     and at the square edges of the offload leg."""
@@ -734,11 +808,77 @@ def cpu_leg(n, ld, target_s, edges=()):
     oracle.set_threads(0)
     if by_edge:
         log(f"cpu baseline by edge: {by_edge}")
-    return {"value": round(rate, 5), "unit": "TFLOP/s", "cores": used, "kind": "port",
+    configs = cpu_configs(oracle, synth) if with_configs else {}
+    return {"value": round(rate, 5), "unit": "TFLOP/s", "cores": used, "kind": "port", "configs": configs,
             "sample": f"DGEMM M={n} N={cols} K={n} (a {cols}-column block of the 8192^3 NN workload, ld={ld}), "
                       f"one call, {dt:.2f}s; restatement of Zynum's portable CPU path (reference not executable: "
                       f"no Zig toolchain); host '{cpu_model}', {os.cpu_count()} logical CPUs",
             "by_edge": by_edge}
+
+
+def cpu_configs(oracle, synth):
+    """The restated CPU path on the other benchmark configurations: the
+    1024^3 cblas_dgemm (alpha 1.5, beta 0.5; best and median of 10 calls after
+    a warm-up), the loop over the 4096 uneven problems (one thread below 192^3
+    as the reference planner, else all), and DTRSM / DSYRK at N = 2048 with
+    the time extrapolated to N = 8192 by (8192 / 2048)^3 (the reference's
+    column-substitution TRSM and dot-product SYRK are cubic)."""
+    from zynum_amd import grouped
+
+    out = {}
+    n = 1024
+    a, b, c0 = synth.matrix(0, 0, n, n, n), synth.matrix(0, n * n, n, n, n), synth.matrix(0, 2 * n * n, n, n, n)
+    oracle.set_threads(0)
+    times = []
+    for r in range(11):
+        c = c0.copy()
+        t0 = oracle.now()
+        oracle.dgemm("N", "N", n, n, n, 1.5, a, n, b, n, 0.5, c, n)
+        if r:
+            times.append(oracle.now() - t0)
+    times.sort()
+    out["cblas_dgemm_1024"] = {"best_us": round(times[0] * 1e6, 1), "median_us": round(times[len(times) // 2] * 1e6, 1),
+                               "best_gflops": round(2.0 * n ** 3 / times[0] / 1e9, 1),
+                               "median_gflops": round(2.0 * n ** 3 / times[len(times) // 2] / 1e9, 1),
+                               "threads": oracle.threads()}
+    del a, b, c0, c
+    shapes = grouped.draw_shapes(4096, 0, 16, 512)
+    rng = np.random.default_rng(0)
+    ops = [(rng.uniform(-1, 1, int(m * k)), rng.uniform(-1, 1, int(k * nn)), np.zeros(int(m * nn)))
+           for m, nn, k in shapes]
+    flop = float(np.sum(2.0 * shapes[:, 0] * shapes[:, 1] * shapes[:, 2]))
+    t0 = oracle.now()
+    for (m, nn, k), (x, y, z) in zip(shapes, ops):
+        oracle.set_threads(1 if m * nn * k < 192 ** 3 else 0)
+        oracle.dgemm("N", "N", int(m), int(nn), int(k), 1.0, x, int(m), y, int(k), 0.0, z, int(m))
+    dt = oracle.now() - t0
+    oracle.set_threads(0)
+    out["grouped_4096_loop"] = {"ms": round(dt * 1e3, 1), "tflops": round(flop / dt / 1e12, 4),
+                                "workload": "4096 DGEMMs NN, M,N,K log-uniform in [16,512] (seed 0), one call each"}
+    del ops
+    n = 2048
+    L = synth.matrix(11, 0, n, n, n).reshape(n, n)
+    L = np.triu(L)  # column-major lower triangle
+    L[np.diag_indices(n)] = 8.0 + np.abs(synth.uniform(11, n * n, n))
+    X = synth.matrix(11, n * n + n, n, n, n)
+    t0 = oracle.now()
+    oracle.dtrsm("L", "L", "N", "N", n, n, 1.0, L.ravel(), n, X, n)
+    dt = oracle.now() - t0
+    out["dtrsm_2048"] = {"ms": round(dt * 1e3, 1), "tflops": round(float(n) ** 3 / dt / 1e12, 4),
+                         "extrapolated_8192_ms": round(dt * 64 * 1e3, 1),
+                         "note": "DTRSM L/L/N/N M=N=2048; 8192 time extrapolated by (8192/2048)^3, not measured"}
+    k = n // 2
+    A = synth.matrix(12, 0, n, k, n)
+    C = np.zeros(n * n)
+    t0 = oracle.now()
+    oracle.dsyrk("L", "N", n, k, 1.0, A, n, 0.0, C, n)
+    dt = oracle.now() - t0
+    out["dsyrk_2048x1024"] = {"ms": round(dt * 1e3, 1), "tflops": round(float(n) * n * k / dt / 1e12, 4),
+                              "extrapolated_8192x4096_ms": round(dt * 64 * 1e3, 1),
+                              "note": "DSYRK Lower NoTrans N=2048 K=1024; 8192x4096 time extrapolated by 4^3, "
+                                      "not measured"}
+    log(f"cpu baseline configs: {out}")
+    return out
 
 
 if __name__ == "__main__":

@@ -280,8 +280,17 @@ int zynum_gpu_dtrmm(char side, char uplo, char transa, char diag, int m, int n, 
 int zynum_gpu_dsyr2k(char uplo, char trans, int n, int k, double alpha, const double *a, int lda,
                      const double *b, int ldb, double beta, double *c, int ldc, void *stream, int flags);
 
-/* Stream used by the Part 1 symbols (default: the null stream). */
+/* Stream used by the Part 1 symbols and by private calls that pass no stream.
+   NULL (the default) selects the library's own stream on the current device:
+   non-blocking, default priority, and ordered after the work already queued
+   on the device's null stream when each call starts.  Synchronous calls wait
+   for their stream before returning. */
 void zynum_gpu_set_stream(void *stream);
+/* Host threads that copy host operands into the pinned staging buffers
+   (1..64; 0 restores the default, min(8, hardware threads / 2)).  Takes
+   effect from the next staged call.  Returns ZYNUM_GPU_OK, or
+   ZYNUM_GPU_DECLINE_ARGUMENT for a count out of range. */
+int zynum_gpu_set_copy_threads(int threads);
 /* Last return code seen by a Part 1 symbol on this thread. */
 int zynum_gpu_last_status(void);
 /* Test hook: the next `count` GPU-tier calls decline with ZYNUM_GPU_DECLINE_FORCED. */

@@ -12,6 +12,7 @@ sys.path.insert(0, str(ROOT / "oracle"))
 
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs a gfx950 GPU (MI355X)")
+    config.addinivalue_line("markers", "perf: wall-clock checks on a GPU, kept out of the gpu parity suite")
 
 
 @pytest.fixture(scope="session")

@@ -226,3 +226,14 @@ def test_typed_view_errors():
         blas.matrix_multiply_with_workspace(A, A, blas.Matrix(np.zeros(4), 2, 2, 2), np.zeros(3))
     assert blas.matrix_multiply_workspace_length(blas.Matrix(np.zeros(12), 3, 4, 3)) == 12
     assert blas.ConstMatrix(np.zeros(6), 2, 3, 2).transposed().effective_row_count() == 3
+
+
+def test_copy_threads_setter_arguments(zlib):
+    """The staging copy-thread count is an API choice (no environment variable): 0 restores the
+    default, 1..64 are taken, anything else declines."""
+    from zynum_amd import _lib
+
+    assert zlib.zynum_gpu_set_copy_threads(4) == _lib.OK
+    assert zlib.zynum_gpu_set_copy_threads(0) == _lib.OK
+    assert zlib.zynum_gpu_set_copy_threads(65) == _lib.DECLINE_ARGUMENT
+    assert zlib.zynum_gpu_set_copy_threads(-1) == _lib.DECLINE_ARGUMENT

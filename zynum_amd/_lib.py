@@ -127,6 +127,7 @@ SIGNATURES = {
     "zynum_gpu_dsyrk": (C.c_int, [C.c_char, C.c_char, C.c_int, C.c_int, C.c_double, C.c_void_p, C.c_int,
                                   C.c_double, C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_int, C.c_int]),
     "zynum_gpu_set_stream": (None, [C.c_void_p]),
+    "zynum_gpu_set_copy_threads": (C.c_int, [C.c_int]),
     "zynum_gpu_last_status": (C.c_int, []),
     "zynum_gpu_debug_force_decline": (None, [C.c_int]),
     "zynum_gpu_status_string": (C.c_char_p, [C.c_int]),

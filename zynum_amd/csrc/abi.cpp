@@ -11,10 +11,14 @@
 
 #include <algorithm>
 
+#include "abi_common.hpp"
 #include "gemm.hpp"
 #include "ops.hpp"
 #include "ozaki.hpp"
 #include "runtime.hpp"
+
+using zg::gemm_error;
+using zg::scalar_from_bits;
 
 namespace {
 
@@ -26,21 +30,6 @@ inline bool valid_uplo(char c) { return c == 'U' || c == 'L'; }
 inline bool valid_diag(char c) { return c == 'N' || c == 'U'; }
 inline bool valid_side(char c) { return c == 'L' || c == 'R'; }
 inline int32_t max1(int32_t x) { return x > 1 ? x : 1; }
-
-// fortran.zig:88-103
-int gemm_error(char ta, char tb, int32_t m, int32_t n, int32_t k, int32_t lda, int32_t ldb, int32_t ldc) {
-    if (!valid_trans(ta)) return 1;
-    if (!valid_trans(tb)) return 2;
-    if (m < 0) return 3;
-    if (n < 0) return 4;
-    if (k < 0) return 5;
-    const int32_t nrowa = (ta == 'N') ? m : k;
-    const int32_t nrowb = (tb == 'N') ? k : n;
-    if (lda < max1(nrowa)) return 8;
-    if (ldb < max1(nrowb)) return 10;
-    if (ldc < max1(m)) return 13;
-    return 0;
-}
 
 // fortran.zig:105-117
 int triangular_error(char side, char uplo, char trans, char diag, int32_t m, int32_t n, int32_t lda, int32_t ldb) {
@@ -108,22 +97,6 @@ void note_status(const char* routine, int st) {
         fprintf(stderr, " ** %s: GPU tier did not run (%s); outputs %s\n", routine, zynum_gpu_status_string(st),
                 st == ZYNUM_GPU_ERR_LAUNCH ? "undefined" : "unchanged");
     }
-}
-
-template <typename T>
-T scalar_from_bits(uint64_t bits);
-template <>
-double scalar_from_bits<double>(uint64_t bits) {
-    double v;
-    memcpy(&v, &bits, sizeof(v));
-    return v;
-}
-template <>
-float scalar_from_bits<float>(uint64_t bits) {
-    uint32_t lo = (uint32_t)bits;
-    float v;
-    memcpy(&v, &lo, sizeof(v));
-    return v;
 }
 
 }  // namespace
@@ -644,9 +617,7 @@ void cblas_dtrsm(int layout, int side, int uplo, int transa, int diag, int m, in
 int zynum_gpu_gemm(const zynum_gpu_gemm_request* r) {
     if (r == nullptr) return ZYNUM_GPU_DECLINE_ARGUMENT;
     const bool cplx = r->scalar == ZYNUM_GPU_C32 || r->scalar == ZYNUM_GPU_C64;
-    auto code = [cplx](uint8_t t) {
-        return t == ZYNUM_GPU_NO_TRANS ? 'N' : t == ZYNUM_GPU_TRANS ? 'T' : t == ZYNUM_GPU_CONJ_TRANS ? (cplx ? 'C' : 'T') : 'X';
-    };
+    auto code = [cplx](uint8_t t) { return zg::request_trans_char(t, cplx); };
     const char ta = code(r->transa), tb = code(r->transb);
     if (gemm_error(ta, tb, r->m, r->n, r->k, r->lda, r->ldb, r->ldc) != 0) return ZYNUM_GPU_DECLINE_ARGUMENT;
     const int tiles = r->mode == ZYNUM_GPU_MODE_PLAIN       ? zg::kTileConfigs

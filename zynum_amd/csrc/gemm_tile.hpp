@@ -113,7 +113,9 @@ constexpr int tile_lds_b() { return 2 * CF::BK * (CF::BN + lds_pad<T>()); }
 // run by all CF::NT threads of the workgroup.  The K loop, the per-element
 // MFMA accumulation order (16x16x4 steps in increasing k) and the epilogue are
 // the same for every tile configuration, so any configuration gives the same
-// bits for the same element.
+// bits for the same element when the whole K range is walked by one tile.
+// (The split-K form sums K ranges separately and then adds the ranges, a
+// different order: bitwise only on inputs where every order is exact.)
 // TRI selects the stored part of C: 0 all of it, 1 the lower triangle
 // (row >= col), 2 the upper triangle (row <= col); other elements are not
 // touched (SYRK on the GEMM core).

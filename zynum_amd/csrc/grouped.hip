@@ -15,7 +15,9 @@
 // loading the descriptor and first K slice of its next tile while the current
 // one finishes, so no atomic or dependent load sits between tiles.  Every tile runs the same MFMA
 // tile routine as the plain GEMM, so each problem's result is bitwise the
-// result of a single plain GEMM call on it.  The launch allocates nothing and
+// result of a single plain GEMM launch on it that walks K in one tile (the
+// single call splits K only for shapes with few output tiles and K >= 1024,
+// and then agrees within the error bound).  The launch allocates nothing and
 // can be captured into a HIP graph.
 #include <string.h>
 

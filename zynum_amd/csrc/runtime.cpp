@@ -114,22 +114,38 @@ class CopyPool {
         return *pool;
     }
 
-    int threads() const { return nthreads_; }
+    // Copy threads in use, the caller included (requested through
+    // zynum_gpu_set_copy_threads; 0 selects the default).
+    static int default_threads() {
+        const unsigned hw = std::thread::hardware_concurrency();
+        return hw > 0 ? std::min(8, (int)std::max(1u, hw / 2)) : 8;
+    }
+    static std::atomic<int>& requested() {
+        static std::atomic<int> r{0};
+        return r;
+    }
+    int threads() const {
+        const int r = requested().load();
+        return r > 0 ? r : default_threads();
+    }
 
-    // Run fn(0) .. fn(parts - 1), spread over the workers and the caller.
-    // A forked child has none of the parent's workers: it copies alone.
+    // Run fn(0) .. fn(parts - 1), spread over threads() - 1 workers and the
+    // caller.  A forked child has none of the parent's workers: it copies alone.
     void run(int parts, const std::function<void(int)>& fn) {
-        if (parts <= 1 || nthreads_ <= 1 || forked_.load()) {
+        const int use = threads();
+        if (parts <= 1 || use <= 1 || forked_.load()) {
             for (int i = 0; i < parts; ++i) fn(i);
             return;
         }
         std::lock_guard<std::mutex> serial(run_mu_);  // one job at a time
+        grow(use - 1);
         {
             std::lock_guard<std::mutex> lock(mu_);
             fn_ = &fn;
             parts_ = parts;
+            active_ = use - 1;
             next_.store(0);
-            pending_ = (int)workers_.size();
+            pending_ = (int)workers_;
             ++generation_;
         }
         cv_.notify_all();
@@ -141,15 +157,16 @@ class CopyPool {
 
   private:
     CopyPool() {
-        int n = 8;
-        const unsigned hw = std::thread::hardware_concurrency();
-        if (hw > 0) n = std::min(n, (int)std::max(1u, hw / 2));
-        if (const char* env = getenv("ZYNUM_GPU_COPY_THREADS")) n = std::max(1, std::min(64, atoi(env)));
-        nthreads_ = n;
         pthread_atfork(nullptr, nullptr, [] { CopyPool::get().forked_.store(true); });
-        for (int i = 1; i < n; ++i) {
-            workers_.emplace_back([this] { loop(); });
-            workers_.back().detach();
+    }
+
+    // Start workers up to `count` (held under run_mu_); they never exit.
+    void grow(int count) {
+        std::lock_guard<std::mutex> lock(mu_);
+        while (workers_ < count) {
+            const int index = workers_++;
+            const uint64_t gen = generation_;  // it takes part from the next job on
+            std::thread([this, index, gen] { loop(index, gen); }).detach();
         }
     }
 
@@ -157,23 +174,24 @@ class CopyPool {
         for (int i = next_.fetch_add(1); i < parts_; i = next_.fetch_add(1)) (*fn_)(i);
     }
 
-    void loop() {
-        uint64_t seen = 0;
+    void loop(int index, uint64_t seen) {
         for (;;) {
+            bool take = false;
             {
                 std::unique_lock<std::mutex> lock(mu_);
                 cv_.wait(lock, [&] { return generation_ != seen; });
                 seen = generation_;
+                take = index < active_;
             }
-            work();
+            if (take) work();
             std::lock_guard<std::mutex> lock(mu_);
             if (--pending_ == 0) done_cv_.notify_one();
         }
     }
 
-    int nthreads_ = 1;
+    int workers_ = 0;  // started so far
+    int active_ = 0;   // workers taking parts of the current job
     std::atomic<bool> forked_{false};
-    std::vector<std::thread> workers_;
     std::mutex run_mu_, mu_;
     std::condition_variable cv_, done_cv_;
     const std::function<void(int)>* fn_ = nullptr;
@@ -262,7 +280,59 @@ bool device_ready() {
     return g_ready;
 }
 
-hipStream_t blas_stream() { return g_blas_stream.load(); }
+namespace {
+// The library's own stream per device: non-blocking and at default priority,
+// so a call does not serialise against unrelated streams of the caller.  It
+// is created on first use and lives until the process ends.
+struct LibStream {
+    hipStream_t stream = nullptr;
+    hipEvent_t after_null = nullptr;  // records the null stream's queue at each call
+};
+constexpr int kLibDevices = 64;
+LibStream g_lib[kLibDevices];
+std::mutex g_lib_mutex;
+
+LibStream* lib_stream_for_current_device() {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kLibDevices) {
+        (void)hipGetLastError();
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lock(g_lib_mutex);
+    LibStream& l = g_lib[dev];
+    if (l.stream == nullptr) {
+        hipStream_t st = nullptr;
+        hipEvent_t ev = nullptr;
+        if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess ||
+            hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) {
+            (void)hipGetLastError();
+            if (ev) (void)hipEventDestroy(ev);
+            if (st) (void)hipStreamDestroy(st);
+            return nullptr;
+        }
+        l.after_null = ev;
+        l.stream = st;
+    }
+    return &l;
+}
+}  // namespace
+
+hipStream_t caller_stream() { return g_blas_stream.load(); }
+
+hipStream_t blas_stream() {
+    hipStream_t s = g_blas_stream.load();
+    if (s != nullptr) return s;
+    // No caller stream: the library stream, ordered after everything queued so
+    // far on the device's null stream (where a caller that names no stream has
+    // written the operands), as a synchronous BLAS call expects.
+    LibStream* l = lib_stream_for_current_device();
+    if (l == nullptr) return nullptr;
+    if (hipEventRecord(l->after_null, nullptr) != hipSuccess || hipStreamWaitEvent(l->stream, l->after_null, 0) != hipSuccess) {
+        (void)hipGetLastError();
+        return nullptr;
+    }
+    return l->stream;
+}
 
 bool take_forced_decline() {
     int v = g_forced.load();
@@ -463,5 +533,11 @@ void workspace_free_all() {
 }  // namespace zg
 
 extern "C" void zynum_gpu_set_stream(void* stream) { zg::g_blas_stream.store(reinterpret_cast<hipStream_t>(stream)); }
+
+extern "C" int zynum_gpu_set_copy_threads(int threads) {
+    if (threads < 0 || threads > 64) return ZYNUM_GPU_DECLINE_ARGUMENT;
+    zg::CopyPool::requested().store(threads);
+    return ZYNUM_GPU_OK;
+}
 
 extern "C" void zynum_gpu_debug_force_decline(int count) { zg::g_forced.store(count > 0 ? count : 0); }

@@ -13,8 +13,11 @@ namespace zg {
 // True when a gfx950 device is present and current (probed once per process).
 bool device_ready();
 
-// Stream used by the classic BLAS symbols (set by zynum_gpu_set_stream).
+// Stream used by the classic BLAS symbols: the one set by
+// zynum_gpu_set_stream, else the library's own stream (created on first use).
 hipStream_t blas_stream();
+// The stream set by zynum_gpu_set_stream (nullptr if none); creates nothing.
+hipStream_t caller_stream();
 
 // Consumes one injected decline; true if the call must decline.
 bool take_forced_decline();
