@@ -170,7 +170,8 @@ enum {
     ZYNUM_GPU_DECLINE_UNSUPPORTED = 3, /* form/mode/scalar not provided */
     ZYNUM_GPU_DECLINE_ARGUMENT = 4,    /* invalid argument (xerbla-style check failed) */
     ZYNUM_GPU_DECLINE_FORCED = 5,      /* decline injected by zynum_gpu_debug_force_decline */
-    ZYNUM_GPU_ERR_LAUNCH = 6           /* HIP error after launch; outputs undefined */
+    ZYNUM_GPU_ERR_LAUNCH = 6,          /* HIP error after launch; outputs undefined */
+    ZYNUM_GPU_DECLINE_SCRATCH = 7      /* zynum_gpu_gemm_noalloc: the call needs more than its fixed scratch */
 };
 
 /* scalar codes of the reference's structured ABI (x86_64_structured_abi.zig:14-19) */
@@ -224,6 +225,21 @@ typedef struct zynum_gpu_gemm_request {
    elements) take CONJ_TRANS as the conjugate transpose and mode PLAIN only; they
    run one real MFMA GEMM on op(A) expanded into 2x2 real blocks. */
 int zynum_gpu_gemm(const zynum_gpu_gemm_request *request);
+
+/* zynum_gpu_gemm for callers that cannot allocate device memory (code under
+   HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+   No counterpart in the reference.  Plain F32 / F64 requests on device (or
+   managed) operands only; other modes, complex scalars and host operands
+   decline with ZYNUM_GPU_DECLINE_UNSUPPORTED.  Results are bitwise those of
+   zynum_gpu_gemm for the same request.  The split-K partials of shapes with
+   few output tiles and a long K live in a fixed 24 MiB device array per GPU
+   (held until the process ends) instead of allocated memory; a call that
+   would need more declines with ZYNUM_GPU_DECLINE_SCRATCH (the split-K plan
+   needs at most 20.9 MB).  Calls no hipMalloc, hipFree or allocating routine
+   and creates no stream: it runs on the request's stream, else the stream
+   set by zynum_gpu_set_stream, else the null stream.  Every call on a GPU
+   shares that GPU's array: calls that overlap on one GPU race. */
+int zynum_gpu_gemm_noalloc(const zynum_gpu_gemm_request *request);
 
 /* One problem of a grouped GEMM batch (device pointers, column-major). */
 typedef struct zynum_gpu_gemm_problem {

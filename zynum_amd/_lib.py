@@ -18,6 +18,7 @@ DECLINE_UNSUPPORTED = 3
 DECLINE_ARGUMENT = 4
 DECLINE_FORCED = 5
 ERR_LAUNCH = 6
+DECLINE_SCRATCH = 7
 
 F32, F64, C32, C64 = 0, 1, 2, 3
 NO_TRANS, TRANS, CONJ_TRANS = 0, 1, 2
@@ -114,6 +115,7 @@ SIGNATURES = {
     "zynum_blas_shutdown": (None, []),
     "zynum_blas_shutdown_": (None, []),
     "zynum_gpu_gemm": (C.c_int, [C.POINTER(GemmRequest)]),
+    "zynum_gpu_gemm_noalloc": (C.c_int, [C.POINTER(GemmRequest)]),
     "zynum_gpu_grouped_plan_create": (C.c_int, [C.c_int, C.POINTER(GemmProblem), C.c_int, C.c_int,
                                                 C.POINTER(C.c_void_p)]),
     "zynum_gpu_grouped_run": (C.c_int, [C.c_void_p, C.c_void_p]),

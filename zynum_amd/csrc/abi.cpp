@@ -707,6 +707,7 @@ const char* zynum_gpu_status_string(int status) {
         case ZYNUM_GPU_DECLINE_ARGUMENT: return "invalid argument";
         case ZYNUM_GPU_DECLINE_FORCED: return "forced decline";
         case ZYNUM_GPU_ERR_LAUNCH: return "HIP launch error";
+        case ZYNUM_GPU_DECLINE_SCRATCH: return "needs more scratch than the fixed no-allocation buffer holds";
         default: return "unknown status";
     }
 }
