@@ -119,11 +119,59 @@ def test_dtrsm_residual_random(torch_cuda, oracle, side, uplo, trans, diag):
     assert np.max(np.abs(X - Xr)) <= 64 * na * 2.0**-53 * cond_scale
 
 
+LEAF_CASES = ["plain", "wide_range", "tiny_column", "huge_diagonal", "zeros", "padded"]
+
+
+@pytest.mark.parametrize("uplo", "LU")
+@pytest.mark.parametrize("case", LEAF_CASES)
+def test_dtrsm_single_leaf_matches_oracle_exactly(torch_cuda, oracle, uplo, case):
+    """One leaf solve (m <= 64, side L, no transpose) performs the reference's
+    column substitution x_k /= a_kk; x_i = fma(a_ik, -x_k, x_i) in the
+    oracle's order, so X equals the oracle's to the last bit, on the
+    reciprocal-based division's range (2^-400 <= |x|, |a_kk| <= 2^400) and
+    outside it (tiny right-hand sides, a huge diagonal: the leaf solves again
+    with the division).  Zeros compare by value: the oracle skips the updates
+    of a zero x_k, which can only change the sign of a zero."""
+    rng = np.random.default_rng(LEAF_CASES.index(case) * 2 + (uplo == "U"))
+    m = 37 if case == "padded" else 64
+    n = 200
+    A = rng.uniform(-1, 1, (m, m))
+    d = rng.uniform(0.5, 2.0, m) * rng.choice([-1.0, 1.0], m)
+    if case == "wide_range":
+        d *= 2.0 ** rng.integers(0, 40, m)
+    if case == "huge_diagonal":
+        d[m // 2] = 1e125  # above 2^400
+    A[np.diag_indices(m)] = d
+    T = np.tril(A) if uplo == "L" else np.triu(A)
+    B = rng.uniform(-1, 1, (m, n))
+    if case == "wide_range":
+        B *= 2.0 ** rng.integers(-350, 350, (m, n))
+    if case == "tiny_column":
+        B[:, 70] *= 1e-150  # below 2^-400: the wave holding column 70 divides
+    if case == "zeros":
+        B[rng.random((m, n)) < 0.3] = 0.0
+        B[:, 3] = -0.0
+    lda, ldb = m + 1, m + 2
+    a = triangle_storage(T, uplo, lda)
+    bb = np.full((n, ldb), -777.0)
+    bb[:, :m] = B.T
+    got = trsm(torch_cuda, "L", uplo, "N", "N", m, n, 1.0, a, lda, bb.ravel(), ldb).reshape(n, ldb)
+    ref = bb.ravel().copy()
+    oracle.dtrsm("L", uplo, "N", "N", m, n, 1.0, np.nan_to_num(a, nan=0.0), lda, ref, ldb)
+    ref = ref.reshape(n, ldb)
+    assert np.all(np.isfinite(ref[:, :m]))
+    assert np.array_equal(got[:, :m], ref[:, :m])
+    if case != "zeros":
+        assert np.array_equal(got.view(np.uint64), ref.view(np.uint64))
+    assert np.all(got[:, m:] == -777.0)
+
+
 @pytest.mark.parametrize("mode,param", [(3, 14), (1, 8)])
 def test_dtrsm_split_precision_updates(torch_cuda, mode, param):
-    """Off-diagonal updates in split precision: exact on integer systems."""
+    """Off-diagonal updates in split precision (the updates whose m, n and k are
+    all at least 1024; smaller ones run on FP64 MFMA): exact on integer systems."""
     rng = np.random.default_rng(3)
-    m, n = 300, 64
+    m, n = 2200, 1100
     T = np.tril(rng.integers(-1, 2, (m, m)).astype(np.float64), -1) + np.eye(m)
     X = rng.integers(-3, 4, (m, n)).astype(np.float64)
     B = T @ X

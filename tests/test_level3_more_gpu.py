@@ -239,6 +239,33 @@ def test_alpha_zero_and_k_zero(torch_cuda):
                           mat(c, 3, 3, 3)[~np.tril(np.ones((3, 3), bool))])
 
 
+@pytest.mark.parametrize("single", [False, True])
+@pytest.mark.parametrize("side", ["L", "R"])
+def test_symm_alpha_zero_forms_the_product(torch_cuda, oracle, single, side):
+    """symmetric.zig:56-70 has no alpha == 0 quick return: 0 * sum is formed, so a
+    NaN or an infinity in the stored triangle of A or in B reaches C, and finite
+    entries give beta C (exact 0 for beta == 0, C not read).  Same as the oracle."""
+    dt = np.float32 if single else np.float64
+    m, n = 37, 29
+    na = m if side == "L" else n
+    rng = np.random.default_rng(3)
+    a = rng.uniform(-1, 1, na * na).astype(dt)
+    a[5 + 7 * na] = np.nan   # stored (lower) triangle, row 5... column 7 lies above the diagonal: unstored
+    a[9 + 2 * na] = np.nan   # row 9, column 2: in the lower triangle
+    b = rng.uniform(-1, 1, m * n).astype(dt)
+    b[3 + 4 * m] = np.inf
+    for beta in (0.0, 2.0):
+        c = rng.uniform(-1, 1, m * n).astype(dt)
+        c[0] = np.nan  # read only when beta != 0
+        got = run_symm(torch_cuda, side, "L", m, n, 0.0, a, na, b, m, beta, c, m, single=single)
+        want = c.astype(np.float64).copy()
+        oracle.dsymm(side, "L", m, n, 0.0, a.astype(np.float64), na, b.astype(np.float64), m, beta, want, m)
+        assert np.array_equal(np.isnan(got), np.isnan(want))
+        fin = ~np.isnan(want)
+        assert np.array_equal(got[fin].astype(np.float64), want[fin])
+        assert np.isnan(got).sum() > 0
+
+
 def test_host_operands_are_staged(zlib):
     """Host (numpy) operands run through the staging path of the same kernels."""
     I_ = C.c_int32

@@ -229,11 +229,9 @@ int symm_op(char side, char uplo, int m, int n, T alpha, const T* a, int64_t lda
     if (!device_ready()) return ZYNUM_GPU_DECLINE_NO_DEVICE;
     const int na = (side == 'L') ? m : n;
     Operand oa, ob, oc;
-    int st = ZYNUM_GPU_OK;
-    if (alpha != T(0)) {
-        st = oa.bind_in(a, na, na, lda, sizeof(T), s);
-        if (st == ZYNUM_GPU_OK) st = ob.bind_in(b, m, n, ldb, sizeof(T), s);
-    }
+    // A and B are read whatever alpha is (symmetric.zig:56-70 has no alpha == 0 quick return)
+    int st = oa.bind_in(a, na, na, lda, sizeof(T), s);
+    if (st == ZYNUM_GPU_OK) st = ob.bind_in(b, m, n, ldb, sizeof(T), s);
     if (st == ZYNUM_GPU_OK) st = oc.bind_out(c, m, n, ldc, sizeof(T), beta != T(0), s);
     if (st != ZYNUM_GPU_OK) return st;
     st = symm_device<T>(side, uplo, m, n, alpha, static_cast<const T*>(oa.dev), lda, static_cast<const T*>(ob.dev), ldb,

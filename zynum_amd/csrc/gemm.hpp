@@ -41,8 +41,14 @@ enum TileConfig : int {
 int pick_tile(int m, int n, int k, int elem_bytes, bool ta, bool tb);
 
 // Launch the plain MFMA GEMM (FP64 or FP32). Returns a hipError_t.
+// k == 0 or alpha == 0 takes the quick return C = beta * C (gemm.zig:170-180).
 template <typename T>
 hipError_t gemm_launch(const GemmDesc<T>& d, hipStream_t stream, int tile = kTileAuto);
+// The same product without the alpha == 0 quick return: alpha * sum is formed,
+// so a NaN or an infinity in A or B still reaches C (for routines whose
+// reference has no such quick return, symmetric.zig:56-70).
+template <typename T>
+hipError_t gemm_launch_product(const GemmDesc<T>& d, hipStream_t stream, int tile = kTileAuto);
 
 // Split-K for few output tiles and a long K: the number of K ranges (1 = do
 // not split) and, through kchunk, their depth.

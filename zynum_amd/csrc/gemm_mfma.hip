@@ -139,6 +139,13 @@ template <typename T>
 hipError_t gemm_launch(const GemmDesc<T>& d, hipStream_t stream, int tile) {
     if (d.m <= 0 || d.n <= 0) return hipSuccess;
     if (d.k <= 0 || d.alpha == T(0)) return scale_launch<T>(d.m, d.n, d.beta, d.c, d.ldc, stream);
+    return gemm_launch_product<T>(d, stream, tile);
+}
+
+template <typename T>
+hipError_t gemm_launch_product(const GemmDesc<T>& d, hipStream_t stream, int tile) {
+    if (d.m <= 0 || d.n <= 0) return hipSuccess;
+    if (d.k <= 0) return scale_launch<T>(d.m, d.n, d.beta, d.c, d.ldc, stream);
     if (tile == kTileAuto) tile = pick_tile(d.m, d.n, d.k, (int)sizeof(T), d.ta, d.tb);
     switch (tile) {
         case kTile64x64: return launch_shape<T, 64, 64, 16, 2, 2, 0>(d, stream);
@@ -212,6 +219,8 @@ template hipError_t gemm_splitk_launch<double>(const GemmDesc<double>&, hipStrea
 template hipError_t gemm_splitk_launch<float>(const GemmDesc<float>&, hipStream_t, float*, int, int);
 template hipError_t gemm_launch<double>(const GemmDesc<double>&, hipStream_t, int);
 template hipError_t gemm_launch<float>(const GemmDesc<float>&, hipStream_t, int);
+template hipError_t gemm_launch_product<double>(const GemmDesc<double>&, hipStream_t, int);
+template hipError_t gemm_launch_product<float>(const GemmDesc<float>&, hipStream_t, int);
 template hipError_t scale_launch<double>(int, int, double, double*, int64_t, hipStream_t);
 template hipError_t scale_launch<float>(int, int, float, float*, int64_t, hipStream_t);
 

@@ -5,10 +5,9 @@
 // in the reference's blocked template (structured_blocked.zig:315-374):
 // split the triangle in two halves, solve the first, update the second with
 // one GEMM (B2 -= op(A)21 X1, plain FP64 MFMA or split precision), recurse.
-// Leaves of at most 64 rows are solved in FP64 from LDS: the 64x64 triangle is
-// staged once per workgroup and each wave64 solves four vectors (columns of B
-// for side L, rows for side R) as interleaved chains with one lane per
-// unknown, in the reference's per-step order x_k /= a_kk;
+// Leaves of at most 64 rows are solved in FP64 with four lanes per vector
+// (column of B for side L, row for side R) and the vector's unknowns in
+// registers, in the reference's per-step order x_k /= a_kk;
 // x_i = fma(a_ik, -x_k, x_i) (matrix_vector/triangular.zig:375-385).
 //
 // SYRK (symmetric.zig:180-279, one strided dot per element) runs as one launch
@@ -18,6 +17,7 @@
 #include <math.h>
 
 #include <algorithm>
+#include <utility>
 
 #include "common.hpp"
 #include "gemm.hpp"
@@ -62,120 +62,228 @@ __device__ __forceinline__ T op_a(const LeafArgs<T>& p, int r, int c) {
     return p.trans ? p.a[(int64_t)r * p.lda + c] : p.a[(int64_t)c * p.lda + r];
 }
 
-// grid.x: groups of kLeafVec vectors; 256 threads = 4 waves, each wave solves
-// kLeafVec / 4 vectors as independent interleaved chains (one lane per
-// unknown), so the per-step shuffle and FMA latencies of one vector overlap
-// with the others'.
-constexpr int kLeafVec = 16;
-
-// Value of v in lane `lane` (wave-uniform) of this wavefront.
-__device__ __forceinline__ double read_lane(double v, int lane) {
-    const uint64_t u = __double_as_longlong(v);
-    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)u, lane);
-    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(u >> 32), lane);
-    return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
-}
-__device__ __forceinline__ float read_lane(float v, int lane) {
-    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
-}
-constexpr int kChains = kLeafVec / 4;
+// Leaf kernel: a quad of lanes per vector, 16 vectors per workgroup (one
+// wave).  The unknowns are numbered in solve order (u = i forward, u = 63 - i
+// backward, so a backward solve is a forward one in u); lane q of a quad holds
+// unknowns u = 4j + q in x[j].  Step k takes the owner lane's (q = k % 4)
+// final x_k to the whole quad with one DPP broadcast, divides it by the
+// diagonal in every lane and eliminates it, one FMA per later unknown of the
+// lane, in the reference's per-column order x_k /= a_kk;
+// x_i = fma(a_ik, -x_k, x_i) (matrix_vector/triangular.zig:375-385).  The
+// coefficients of a step are laid out in LDS per lane position,
+// Cs[k][q][j] = a_{4j+q, k}, and read two steps ahead of their use; the
+// diagonals are read as broadcasts.  A lane does a quarter of the FMAs and
+// reads a quarter of the coefficients of the one-lane-per-vector form, and
+// the grid has four times the waves: with one wave per SIMD the solve is
+// bound by instruction issue, and the per-step division, which every lane
+// executes, is then the largest share.  Unknowns past nb are padding:
+// diagonal 1 and zero coefficients, so they stay 0 and eliminate nothing.
+constexpr int kQuad = 4;                  // lanes per vector
+constexpr int kLeafU = kLeaf / kQuad;     // unknowns per lane
+constexpr int kLeafThreads = 64;
+constexpr int kLeafVecs = kLeafThreads / kQuad;  // vectors per workgroup
+constexpr int kLeafAhead = 2;             // steps whose coefficients are read ahead
+constexpr int kLdsQ = kLeafU + 2;         // per-lane-position row (odd in 16-byte units: no bank conflict)
+constexpr int kLdsK = kQuad * kLdsQ;      // per-step row
 
 template <typename T>
-__global__ __launch_bounds__(256) void trsm_leaf_kernel(LeafArgs<T> p) {
-    __shared__ T M[kLeaf][kLeaf + 1];     // M[i][k]: coefficient of unknown k in equation i
-    __shared__ T V[kLeafVec][kLeaf + 1];  // V[v][i]: vector v, unknown i
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int v0 = blockIdx.x * kLeafVec;
+__device__ __forceinline__ T quad_bcast(T v, int src);
+template <>
+__device__ __forceinline__ double quad_bcast<double>(double v, int src) {
+    // DPP quad_perm [src, src, src, src] on both halves; src is a constant at every call
+    const uint64_t u = __double_as_longlong(v);
+    int lo = (int)(uint32_t)u, hi = (int)(uint32_t)(u >> 32);
+    switch (src) {
+        case 0: lo = __builtin_amdgcn_mov_dpp(lo, 0x00, 0xf, 0xf, false); hi = __builtin_amdgcn_mov_dpp(hi, 0x00, 0xf, 0xf, false); break;
+        case 1: lo = __builtin_amdgcn_mov_dpp(lo, 0x55, 0xf, 0xf, false); hi = __builtin_amdgcn_mov_dpp(hi, 0x55, 0xf, 0xf, false); break;
+        case 2: lo = __builtin_amdgcn_mov_dpp(lo, 0xaa, 0xf, 0xf, false); hi = __builtin_amdgcn_mov_dpp(hi, 0xaa, 0xf, 0xf, false); break;
+        default: lo = __builtin_amdgcn_mov_dpp(lo, 0xff, 0xf, 0xf, false); hi = __builtin_amdgcn_mov_dpp(hi, 0xff, 0xf, 0xf, false); break;
+    }
+    return __longlong_as_double((long long)(((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo));
+}
+template <>
+__device__ __forceinline__ float quad_bcast<float>(float v, int src) {
+    int x = __float_as_int(v);
+    switch (src) {
+        case 0: x = __builtin_amdgcn_mov_dpp(x, 0x00, 0xf, 0xf, false); break;
+        case 1: x = __builtin_amdgcn_mov_dpp(x, 0x55, 0xf, 0xf, false); break;
+        case 2: x = __builtin_amdgcn_mov_dpp(x, 0xaa, 0xf, 0xf, false); break;
+        default: x = __builtin_amdgcn_mov_dpp(x, 0xff, 0xf, 0xf, false); break;
+    }
+    return __int_as_float(x);
+}
+
+// -v as an opaque register value: otherwise the negation is sunk into each of
+// the FMAs that use it, one code-generation pass at a time (minutes of
+// compile time for the same instructions)
+template <typename T>
+__device__ __forceinline__ T neg_opaque(T v) {
+    T n = -v;
+    asm volatile("" : "+v"(n));
+    return n;
+}
+
+// coefficients (and diagonal) of step K into ring slot K % (kLeafAhead + 1);
+// cq points at this lane's Cs[0][q][0]
+template <typename T, bool UNIT, int K>
+__device__ __forceinline__ void leaf_fetch(T (&c)[kLeafAhead + 1][kLeafU], T (&d)[kLeafAhead + 1], const T* cq,
+                                           const T* Dg) {
+    if constexpr (K < kLeaf) {
+        constexpr int slot = K % (kLeafAhead + 1), j0 = K / kQuad;
+#pragma unroll
+        for (int j = j0; j < kLeafU; ++j) c[slot][j] = cq[K * kLdsK + j];
+        if constexpr (!UNIT) d[slot] = Dg[K];
+    }
+}
+
+template <typename T, bool UNIT, int K>
+__device__ __forceinline__ void leaf_step(T (&x)[kLeafU], T (&c)[kLeafAhead + 1][kLeafU], T (&d)[kLeafAhead + 1],
+                                          const T* cq, const T* Dg, int q) {
+    leaf_fetch<T, UNIT, K + kLeafAhead>(c, d, cq, Dg);
+    constexpr int slot = K % (kLeafAhead + 1), jo = K / kQuad, o = K % kQuad;
+    T xk = quad_bcast<T>(x[jo], o);  // x_k from its owner lane
+    if constexpr (!UNIT) xk = xk / d[slot];
+    const T nxk = neg_opaque(xk);
+#pragma unroll
+    for (int j = jo + 1; j < kLeafU; ++j) x[j] = fma_t(c[slot][j], nxk, x[j]);
+    // slot jo: unknown k in the owner, earlier (solved) unknowns below it,
+    // later ones above
+    // (unpredictable: selects, not branches around the FMA and the division)
+    const T upd = fma_t(c[slot][jo], nxk, x[jo]);
+    const T keep = __builtin_unpredictable(q > o) ? upd : x[jo];
+    x[jo] = __builtin_unpredictable(q == o) ? xk : keep;
+    // one scheduling region per step: the reads stay ahead, register pressure
+    // and compile time stay bounded
+    __builtin_amdgcn_sched_barrier(0);
+}
+
+template <typename T, bool UNIT, int... K>
+__device__ __forceinline__ void leaf_solve(T (&x)[kLeafU], const T* cq, const T* Dg, int q,
+                                           std::integer_sequence<int, K...>) {
+    T c[kLeafAhead + 1][kLeafU];
+    T d[kLeafAhead + 1];
+#pragma unroll
+    for (int s = 0; s < kLeafAhead; ++s) {
+#pragma unroll
+        for (int j = s / kQuad; j < kLeafU; ++j) c[s][j] = cq[s * kLdsK + j];
+        if constexpr (!UNIT) d[s] = Dg[s];
+    }
+    (leaf_step<T, UNIT, K>(x, c, d, cq, Dg, q), ...);
+}
+
+template <typename T, bool FWD, bool UNIT>
+__global__ __launch_bounds__(kLeafThreads) void trsm_leaf_kernel(LeafArgs<T> p) {
+    __shared__ __attribute__((aligned(16))) T Cs[kLeaf * kLdsK];  // Cs[k][q][j] = a_{4j+q, k} (solve order)
+    __shared__ T Dg[kLeaf];                      // diagonal in solve order (1 for padding)
+    __shared__ T Xs[kLeafVecs * (kLeaf + 1)];    // side L: vectors staged for the transposition
+    const int t = threadIdx.x, q = t & (kQuad - 1), vv = t / kQuad;
+    const int v0 = blockIdx.x * kLeafVecs, v = v0 + vv;  // this quad's vector
     const int nb = p.nb;
-    // Equation matrix: side L solves op(A) x = b  (M = op(A) block),
-    // side R solves x op(A) = b, i.e. op(A)^T x^T = b^T (M = op(A)^T block).
-    // M is lower iff (side L and op(A) lower) or (side R and op(A) upper).
-    const bool m_lower = p.left ? p.lower_eff : !p.lower_eff;
-    // All loads of a thread are issued before any is used: indices are
-    // clamped into the block (always in bounds) and the unused values are
-    // replaced afterwards, so no load waits behind another one's branch.
-    constexpr int kMLoads = kLeaf * kLeaf / 256, kVLoads = kLeafVec * kLeaf / 256;
-    T mv[kMLoads];
+    const int tc = min(t, nb - 1);
+    auto idx = [](int u) { return FWD ? u : kLeaf - 1 - u; };  // solve order -> index in the block
+    // Every load of a phase is issued before the first is used (one memory
+    // round trip per phase).  Triangle block, read column by column of the
+    // stored matrix (thread t takes stored row i0 + t, so the wave reads one
+    // contiguous column).  A full block (nb = 64, every leaf but possibly the
+    // last) needs no clamping and no zeroing: every slot a step reads for a
+    // later unknown holds its coefficient, and the other slots only feed
+    // updates that the owner selection discards.
+    const bool full = (nb == kLeaf);
+    const T* ablk = p.a + (int64_t)p.i0 * p.lda + p.i0;
+    T av[kLeaf];
+    if (full) {
+        const T* col = ablk + t;
 #pragma unroll
-    for (int t = 0; t < kMLoads; ++t) {
-        const int e = tid + 256 * t;
-        const int i = min(e % kLeaf, nb - 1), k = min(e / kLeaf, nb - 1);
-        mv[t] = p.left ? op_a(p, p.i0 + i, p.i0 + k) : op_a(p, p.i0 + k, p.i0 + i);
+        for (int c = 0; c < kLeaf; ++c, col += p.lda) av[c] = *col;
+    } else {
+#pragma unroll
+        for (int c = 0; c < kLeaf; ++c) av[c] = ablk[(int64_t)min(c, nb - 1) * p.lda + tc];
     }
-    T bv[kVLoads];
+    const T dv = ablk[(int64_t)tc * p.lda + tc];  // diagonal of equation t
+    T x[kLeafU];
+    if (p.left) {
+        // vector v is column v of B, unknown i row i0 + i: the wave reads
+        // columns (contiguous rows), transposed through LDS below
+        T bv[kLeafVecs];
+        const T* bblk = p.b + p.i0 + tc;
+        if (v0 + kLeafVecs <= p.nvec) {
+            const T* col = bblk + (int64_t)v0 * p.ldb;
 #pragma unroll
-    for (int t = 0; t < kVLoads; ++t) {
-        const int e = tid + 256 * t;
-        int v, i;
-        if (p.left) { i = e % kLeaf; v = e / kLeaf; } else { v = e % kLeafVec; i = e / kLeafVec; }
-        i = min(i, nb - 1);
-        v = min(v0 + v, p.nvec - 1);
-        bv[t] = p.left ? p.b[(int64_t)v * p.ldb + p.i0 + i] : p.b[(int64_t)(p.i0 + i) * p.ldb + v];
-    }
+            for (int w = 0; w < kLeafVecs; ++w, col += p.ldb) bv[w] = *col;
+        } else {
 #pragma unroll
-    for (int t = 0; t < kMLoads; ++t) {
-        const int e = tid + 256 * t;
-        const int i = e % kLeaf, k = e / kLeaf;
-        T v = T(0);
-        if (i < nb && k < nb) {
-            if (i == k) v = p.unit ? T(1) : mv[t];
-            else if (m_lower ? (i > k) : (i < k)) v = mv[t];
+            for (int w = 0; w < kLeafVecs; ++w) bv[w] = bblk[(int64_t)min(v0 + w, p.nvec - 1) * p.ldb];
         }
-        M[i][k] = v;
-    }
-    // vectors: side L, vector v = column v0 + v of B, unknown i = row i0 + i;
-    // side R, vector v = row v0 + v, unknown i = column i0 + i
 #pragma unroll
-    for (int t = 0; t < kVLoads; ++t) {
-        const int e = tid + 256 * t;
-        int v, i;
-        if (p.left) { i = e % kLeaf; v = e / kLeaf; } else { v = e % kLeafVec; i = e / kLeafVec; }
-        V[v][i] = (i < nb && v0 + v < p.nvec) ? bv[t] : T(0);
+        for (int w = 0; w < kLeafVecs; ++w) Xs[w * (kLeaf + 1) + t] = bv[w];
+    } else {
+        // vector v is row v of B, unknown i column i0 + i
+        const T* bvec = p.b + (int64_t)p.i0 * p.ldb + min(v, p.nvec - 1);
+#pragma unroll
+        for (int j = 0; j < kLeafU; ++j) x[j] = bvec[(int64_t)min(idx(kQuad * j + q), nb - 1) * p.ldb];
+    }
+    if constexpr (!UNIT) Dg[FWD ? t : kLeaf - 1 - t] = (t < nb) ? dv : T(1);
+    // Equation matrix: side L solves op(A) x = b (M = op(A) block), side R
+    // solves x op(A) = b, i.e. op(A)^T x^T = b^T (M = op(A)^T block), so the
+    // stored element (row r, column c) is M[r][c] for side L without
+    // transpose and for side R with it, and M[c][r] otherwise.  Every
+    // (equation, unknown) pair has its own slot.
+    auto slot = [](int i, int k) {  // slot of M[i][k]
+        const int ui = FWD ? i : kLeaf - 1 - i, uk = FWD ? k : kLeaf - 1 - k;
+        return uk * kLdsK + (ui % kQuad) * kLdsQ + ui / kQuad;
+    };
+    auto keep = [&](int i, int k) {  // a coefficient some step reads
+        const int ui = FWD ? i : kLeaf - 1 - i, uk = FWD ? k : kLeaf - 1 - k;
+        return i < nb && k < nb && ui > uk;
+    };
+    const bool swap = (p.left != 0) == (p.trans != 0);
+    if (full) {
+        if (swap) {
+#pragma unroll
+            for (int c = 0; c < kLeaf; ++c) Cs[slot(c, t)] = av[c];
+        } else {
+#pragma unroll
+            for (int c = 0; c < kLeaf; ++c) Cs[slot(t, c)] = av[c];
+        }
+    } else {
+        if (swap) {
+#pragma unroll
+            for (int c = 0; c < kLeaf; ++c) Cs[slot(c, t)] = keep(c, t) ? av[c] : T(0);
+        } else {
+#pragma unroll
+            for (int c = 0; c < kLeaf; ++c) Cs[slot(t, c)] = keep(t, c) ? av[c] : T(0);
+        }
     }
     __syncthreads();
-    T x[kChains];
+    if (p.left) {
 #pragma unroll
-    for (int c = 0; c < kChains; ++c) x[c] = V[wave * kChains + c][lane];
-    // Step k: unknown k of every chain is read from lane k into scalar
-    // registers (k is wave-uniform), divided by the diagonal (lane c of one
-    // vector division holds chain c's numerator, so all chains share one
-    // correctly rounded division), kept by lane k and eliminated from the
-    // equations of the other lanes.  The chains are updated with selects, not
-    // branches.
-    const bool unit = p.unit;
-    for (int step = 0; step < nb; ++step) {
-        const int k = m_lower ? step : nb - 1 - step;
-        const T mk = M[lane][k];
-        const T dk = M[k][k];
-        const bool below = m_lower ? (lane > k && lane < nb) : (lane < k);
-        T xk[kChains];
-#pragma unroll
-        for (int c = 0; c < kChains; ++c) xk[c] = read_lane(x[c], k);
-        if (!unit) {
-            // one division for all chains: lane c divides chain c's value
-            T num = xk[0];
-#pragma unroll
-            for (int c = 1; c < kChains; ++c) num = (lane == c) ? xk[c] : num;
-            const T quo = num / dk;
-#pragma unroll
-            for (int c = 0; c < kChains; ++c) xk[c] = read_lane(quo, c);
-        }
-#pragma unroll
-        for (int c = 0; c < kChains; ++c) {
-            const T upd = fma_t(mk, -xk[c], x[c]);
-            x[c] = (lane == k) ? xk[c] : (below ? upd : x[c]);
-        }
+        for (int j = 0; j < kLeafU; ++j) x[j] = Xs[vv * (kLeaf + 1) + idx(kQuad * j + q)];
     }
 #pragma unroll
-    for (int c = 0; c < kChains; ++c) V[wave * kChains + c][lane] = x[c];
-    __syncthreads();
-    for (int e = tid; e < kLeafVec * kLeaf; e += 256) {
-        int v, i;
-        if (p.left) { i = e % kLeaf; v = e / kLeaf; } else { v = e % kLeafVec; i = e / kLeafVec; }
-        if (i < nb && v0 + v < p.nvec) {
-            if (p.left) p.b[(int64_t)(v0 + v) * p.ldb + p.i0 + i] = V[v][i];
-            else p.b[(int64_t)(p.i0 + i) * p.ldb + v0 + v] = V[v][i];
+    for (int j = 0; j < kLeafU; ++j)
+        if (idx(kQuad * j + q) >= nb) x[j] = T(0);
+    leaf_solve<T, UNIT>(x, Cs + q * kLdsQ, Dg, q, std::make_integer_sequence<int, kLeaf>{});
+    if (p.left) {
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < kLeafU; ++j) Xs[vv * (kLeaf + 1) + idx(kQuad * j + q)] = x[j];
+        __syncthreads();
+        T* bblk = p.b + p.i0 + t;
+        if (t < nb) {
+            if (v0 + kLeafVecs <= p.nvec) {
+                T* col = bblk + (int64_t)v0 * p.ldb;
+#pragma unroll
+                for (int w = 0; w < kLeafVecs; ++w, col += p.ldb) *col = Xs[w * (kLeaf + 1) + t];
+            } else {
+#pragma unroll
+                for (int w = 0; w < kLeafVecs; ++w)
+                    if (v0 + w < p.nvec) bblk[(int64_t)(v0 + w) * p.ldb] = Xs[w * (kLeaf + 1) + t];
+            }
         }
+    } else if (v < p.nvec) {
+#pragma unroll
+        for (int j = 0; j < kLeafU; ++j)
+            if (idx(kQuad * j + q) < nb) p.b[(int64_t)(p.i0 + idx(kQuad * j + q)) * p.ldb + v] = x[j];
     }
 }
 
@@ -201,12 +309,22 @@ const T* sub_opa(const TrsmCtx<T>& c, int r0, int c0) {
     return c.trans ? c.a + r0 * c.lda + c0 : c.a + c0 * c.lda + r0;
 }
 
+// smallest m, n and k of an off-diagonal update that runs in split precision
+#ifndef ZG_SPLIT_UPDATE_MIN
+#define ZG_SPLIT_UPDATE_MIN 1024
+#endif
+constexpr int kSplitUpdateMin = ZG_SPLIT_UPDATE_MIN;
+
 template <typename T>
 int gemm_update(const TrsmCtx<T>& c, bool ta, bool tb, int m, int n, int k, const T* a, int64_t lda, const T* b,
                 int64_t ldb, T* cc, int64_t ldc) {
     if (m <= 0 || n <= 0 || k <= 0) return ZYNUM_GPU_OK;
     if constexpr (sizeof(T) == 8) {
-        if (c.mode != ZYNUM_GPU_MODE_PLAIN)
+        // Split precision pays only on large updates: its operand preparation,
+        // reconstruction and per-call synchronisation are fixed costs that the
+        // small updates at the bottom of the recursion cannot amortise, so
+        // those run on FP64 MFMA (same or smaller error).
+        if (c.mode != ZYNUM_GPU_MODE_PLAIN && std::min(std::min(m, n), k) >= kSplitUpdateMin)
             return ozaki_dgemm(ta, tb, m, n, k, -1.0, a, lda, b, ldb, 1.0, cc, ldc, c.s, c.mode, c.mode_param);
     }
     GemmDesc<T> d;
@@ -224,8 +342,16 @@ int leaf(const TrsmCtx<T>& c, int i0, int nb) {
     p.left = c.left; p.lower_eff = c.lower_eff; p.trans = c.trans; p.unit = c.unit;
     p.i0 = i0; p.nb = nb;
     p.nvec = c.left ? c.n : c.m;
-    const int grid = (p.nvec + kLeafVec - 1) / kLeafVec;
-    trsm_leaf_kernel<T><<<grid, 256, 0, c.s>>>(p);
+    const int grid = (p.nvec + kLeafVecs - 1) / kLeafVecs;
+    // forward substitution when the equations are lower triangular
+    const bool fwd = c.left ? c.lower_eff : !c.lower_eff;
+    if (fwd) {
+        if (c.unit) trsm_leaf_kernel<T, true, true><<<grid, kLeafThreads, 0, c.s>>>(p);
+        else trsm_leaf_kernel<T, true, false><<<grid, kLeafThreads, 0, c.s>>>(p);
+    } else {
+        if (c.unit) trsm_leaf_kernel<T, false, true><<<grid, kLeafThreads, 0, c.s>>>(p);
+        else trsm_leaf_kernel<T, false, false><<<grid, kLeafThreads, 0, c.s>>>(p);
+    }
     return status_from_hip(hipGetLastError());
 }
 
@@ -473,6 +599,8 @@ __global__ void trmm_fill_kernel(int n, int lower, int unit, const T* __restrict
 // (side R, S n x n), S symmetric from the stored triangle of A.  The reference
 // (symmetric.zig:56-70) forms each element as one sum over the full row of S;
 // here S is expanded once into the workspace and the tuned GEMM runs on it.
+// The reference has no alpha == 0 quick return (alpha * sum is formed, so a
+// NaN or an infinity in S or B reaches C): neither has this product.
 template <typename T>
 int symm_device(char side, char uplo, int m, int n, T alpha, const T* a, int64_t lda, const T* b, int64_t ldb, T beta,
                 T* c, int64_t ldc, hipStream_t s) {
@@ -482,10 +610,6 @@ int symm_device(char side, char uplo, int m, int n, T alpha, const T* a, int64_t
     d.m = m; d.n = n;
     d.alpha = alpha; d.beta = beta;
     d.c = c; d.ldc = ldc;
-    if (alpha == T(0)) {  // C = beta * C (exact 0 for beta == 0)
-        d.k = 0;
-        return status_from_hip(gemm_launch<T>(d, s));
-    }
     const int na = left ? m : n;
     const int64_t lds = ld_round(na);
     T* sym = static_cast<T*>(workspace_acquire(bytes_round((size_t)lds * na * sizeof(T))));
@@ -505,7 +629,7 @@ int symm_device(char side, char uplo, int m, int n, T alpha, const T* a, int64_t
         } else {
             d.k = n; d.a = b; d.lda = ldb; d.b = sym; d.ldb = lds;
         }
-        e = gemm_launch<T>(d, s);
+        e = gemm_launch_product<T>(d, s);
     }
     int st = status_from_hip(e);
     if (st == ZYNUM_GPU_OK) st = status_from_hip(hipStreamSynchronize(s));  // workspace reuse is host-ordered
