@@ -215,7 +215,7 @@ typedef struct zynum_gpu_gemm_request {
     uint8_t mode_param;  /* slices or moduli count; 0 picks the default */
     uint8_t flags;       /* ZYNUM_GPU_FLAG_* */
     uint8_t tile;        /* tile configuration of the GEMM core: 0 = tuned choice, i + 1 = configuration i
-                            (plain: FP64/FP32 MFMA tiles 0..11; INT8 modes: 0..13; BF16: 0..3) */
+                            (plain: FP64/FP32 MFMA tiles 0..11; INT8 modes: 0..25; BF16: 0..3) */
     uint8_t reserved1[4];
 } zynum_gpu_gemm_request; /* 104 bytes */
 

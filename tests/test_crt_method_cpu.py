@@ -145,3 +145,49 @@ def test_single_reduction_step_is_not_enough_at_20_moduli():
             if sym_mod(v, m, two_step=False) != exact_sym_mod(v, m):
                 bad += 1
     assert bad > 0
+
+
+def balanced(v, m):
+    r = v % m
+    return r - m if r > (m - 1) // 2 else r
+
+
+def test_two_step_odd_residue_is_exact():
+    """The plane kernel's residue of an integer-valued double v for an odd
+    modulus m: t = v - m rint(v inv_m), then r = t - m rint(t inv_m), both
+    differences exact (fma of integers with an integer result).  Emulated with
+    IEEE doubles for the products and roundings and exact integers for the
+    differences, r is the balanced residue in [-(m-1)/2, (m-1)/2] for every v up
+    to 2^100 in magnitude, including values next to multiples of m and to
+    half-multiples, where a one-step rounding of v / m can go either way."""
+    rnd = random.Random(17)
+    cases = []
+    for m in MODULI[1:]:  # the odd moduli
+        inv = 1.0 / m
+        for bits in (8, 30, 53, 56, 78, 100):
+            for _ in range(400):
+                v = rnd.randrange(-(1 << bits), 1 << bits)
+                if bits > 53:
+                    v = int(float(v))  # an integer-valued double
+                cases.append((v, m, inv))
+            for k in (1, 3, 12345, (1 << 40) + 7):
+                for d in (-1, 0, 1):
+                    cases.append((k * m + d, m, inv))
+                    cases.append((-(k * m) + d, m, inv))
+                    cases.append((k * m + (m - 1) // 2 + d, m, inv))
+    for v, m, inv in cases:
+        t = v - m * int(round(float(v) * inv))  # rint = round half to even, as Python's round on floats
+        r = t - m * int(round(float(t) * inv))
+        assert r == balanced(v, m), (v, m, t, r)
+
+
+def test_one_step_residue_needs_the_second_step():
+    """A plausible shortcut, one rounded quotient, leaves |r| above (m-1)/2 for large v."""
+    rnd = random.Random(3)
+    m, inv = 173, 1.0 / 173
+    bad = 0
+    for _ in range(2000):
+        v = int(float(rnd.randrange(-(1 << 78), 1 << 78)))
+        r = v - m * int(round(float(v) * inv))
+        bad += abs(r) > (m - 1) // 2
+    assert bad > 0

@@ -181,7 +181,7 @@ def test_crt_integer_exact_shapes(torch_cuda, shape, ta, tb):
     assert np.array_equal(got, want), mismatch_report(got, want)
 
 
-I8_TILES = 14
+I8_TILES = 26
 BF16_TILES = 4
 
 

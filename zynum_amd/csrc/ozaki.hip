@@ -185,27 +185,43 @@ __global__ __launch_bounds__(256) void row_stats_kernel(const double* __restrict
         atomicMax((unsigned long long*)&maxbits[r], (unsigned long long)__double_as_longlong(mx));
         ps[r] = ss;
     } else {
+        // one wave per row, lanes over k; four rows at a time so that their
+        // 16 loads per lane are in flight together (indices clamped into the
+        // chunk and the row range, the extra values masked)
         const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-        for (int rr = wave; rr < 64; rr += 4) {
-            const int r = blockIdx.x * 64 + rr;
-            if (r >= rows) break;
-            double mx = 0.0, ss = 0.0;
-            bool nan = false;
-            for (int kk = k0 + lane; kk < k1; kk += 64) {
-                const double v = x[kk + (int64_t)r * ld];
-                mx = fmax(mx, fabs(v));
-                nan |= (v != v);
-                ss = fma(v, v, ss);
+        constexpr int kRowsTogether = 4, kPer = kStatChunk / 64;
+        for (int rr0 = wave * kRowsTogether; rr0 < 64; rr0 += 4 * kRowsTogether) {
+            double v[kRowsTogether][kPer];
+#pragma unroll
+            for (int q = 0; q < kRowsTogether; ++q) {
+                const int r = min(blockIdx.x * 64 + rr0 + q, rows - 1);
+#pragma unroll
+                for (int t = 0; t < kPer; ++t) v[q][t] = x[min(k0 + lane + 64 * t, k1 - 1) + (int64_t)r * ld];
             }
-            for (int off = 32; off > 0; off >>= 1) {
-                mx = fmax(mx, __shfl_xor(mx, off));
-                nan |= (bool)__shfl_xor((int)nan, off);
-                ss += __shfl_xor(ss, off);
-            }
-            if (nan) mx = __longlong_as_double(0x7ff8000000000000ll);
-            if (lane == 0) {
-                atomicMax((unsigned long long*)&maxbits[r], (unsigned long long)__double_as_longlong(mx));
-                ps[r] = ss;
+#pragma unroll
+            for (int q = 0; q < kRowsTogether; ++q) {
+                const int r = blockIdx.x * 64 + rr0 + q;
+                double mx = 0.0, ss = 0.0;
+                bool nan = false;
+#pragma unroll
+                for (int t = 0; t < kPer; ++t) {
+                    if (k0 + lane + 64 * t < k1) {
+                        const double w = v[q][t];
+                        mx = fmax(mx, fabs(w));
+                        nan |= (w != w);
+                        ss = fma(w, w, ss);
+                    }
+                }
+                for (int off = 32; off > 0; off >>= 1) {
+                    mx = fmax(mx, __shfl_xor(mx, off));
+                    nan |= (bool)__shfl_xor((int)nan, off);
+                    ss += __shfl_xor(ss, off);
+                }
+                if (nan) mx = __longlong_as_double(0x7ff8000000000000ll);
+                if (lane == 0 && r < rows) {
+                    atomicMax((unsigned long long*)&maxbits[r], (unsigned long long)__double_as_longlong(mx));
+                    ps[r] = ss;
+                }
             }
         }
     }
@@ -283,6 +299,17 @@ __device__ __forceinline__ int sym_mod(double v, int m, double inv_m) {
     return ri;
 }
 
+// Balanced residue for an odd modulus, in two exact reduction steps and no
+// integer fix-ups: the first leaves an integer |t| <= m/2 + |v| 2^-52 < 2^48
+// (as sym_mod argues); the second rounds t / m correctly, because the
+// computed t inv_m is within |t / m| 2^-51 < 2^-10 of t / m while, m being
+// odd, t / m is at least 1 / (2m) >= 2^-9 away from every half-integer.  So
+// r = t - m rint(t / m) lies in [-(m-1)/2, (m-1)/2]: the value sym_mod returns.
+__device__ __forceinline__ int sym_mod_odd(double v, int m, double inv_m) {
+    const double t = fma(-rint(v * inv_m), (double)m, v);
+    return (int)fma(-rint(t * inv_m), (double)m, t);
+}
+
 // Rows of op(X) per block of the plane kernel; each of the 256 threads then
 // converts EPT = TR * 128 / 256 consecutive k of one row.
 #ifndef ZG_PLANE_ROWS
@@ -340,7 +367,7 @@ __global__ __launch_bounds__(256) void plane_kernel(PlaneArgs p) {
                 uint32_t word = 0;
 #pragma unroll
                 for (int b = 0; b < 4; ++b) {
-                    const int res = sym_mod(a[4 * w + b], ml, iml);
+                    const int res = (ml & 1) ? sym_mod_odd(a[4 * w + b], ml, iml) : sym_mod(a[4 * w + b], ml, iml);
                     word |= ((uint32_t)(uint8_t)(int8_t)res) << (8 * b);
                 }
                 pw[w] = word;
@@ -530,7 +557,10 @@ __device__ __forceinline__ void i8_store(const I8Args& p, const I8Out& out, int 
 #define ZG_I8_LDS_PAD 16
 #endif
 
-template <int BM, int BN, int BK, int WM, int WN, int OCC, class MF>
+// SWZ: rows are unpadded and the 16-byte chunk c of row r is stored at chunk
+// c ^ (r % 8) (BK of 128 bytes: 8 chunks per row), so the 16-byte reads of 8
+// consecutive rows at one k offset, and the writes of one row, cover all banks.
+template <int BM, int BN, int BK, int WM, int WN, int OCC, class MF, bool SWZ = false>
 __global__ __launch_bounds__(64 * WM * WN) __attribute__((amdgpu_waves_per_eu(OCC)))
 void i8_gemm_kernel(I8Args p) {
     constexpr int MS = MF::MSZ;
@@ -538,8 +568,11 @@ void i8_gemm_kernel(I8Args p) {
     constexpr int kFlush = 8192 / BK;
     typedef typename MF::acc_t acc_t;
     constexpr int NT = 64 * WM * WN;
-    constexpr int RS = BK + ZG_I8_LDS_PAD;  // padded LDS row (bytes): 16-byte reads of consecutive rows hit distinct banks
+    constexpr int RS = SWZ ? BK : BK + ZG_I8_LDS_PAD;  // LDS row (bytes)
     constexpr int CPR = BK / 16;     // 16-byte chunks per row
+    static_assert(!SWZ || CPR == 8, "the swizzle is laid out for 128-byte rows");
+    // byte offset of 16-byte chunk `c16` (a multiple of 16) of row `row`
+    auto at = [](int row, int c16) { return row * RS + (SWZ ? (c16 ^ ((row & 7) << 4)) : c16); };
     constexpr int WTM = BM / WM, WTN = BN / WN;
     constexpr int FM = WTM / MS, FN = WTN / MS;
     constexpr int LA = BM * CPR / NT;  // 16-byte loads per thread (A)
@@ -612,12 +645,12 @@ void i8_gemm_kernel(I8Args p) {
 #pragma unroll
         for (int i = 0; i < LA; ++i) {
             const int e = tid + NT * i, row = e / CPR, c = (e % CPR) * 16;
-            *reinterpret_cast<int4*>(stage_a(buf) + row * RS + c) = ra[i];
+            *reinterpret_cast<int4*>(stage_a(buf) + at(row, c)) = ra[i];
         }
 #pragma unroll
         for (int i = 0; i < LB; ++i) {
             const int e = tid + NT * i, row = e / CPR, c = (e % CPR) * 16;
-            *reinterpret_cast<int4*>(stage_b(buf) + row * RS + c) = rb[i];
+            *reinterpret_cast<int4*>(stage_b(buf) + at(row, c)) = rb[i];
         }
     };
 
@@ -641,12 +674,12 @@ void i8_gemm_kernel(I8Args p) {
             // order, so the LDS latency hides behind FN MFMAs)
             v4i bf[FN];
 #pragma unroll
-            for (int j = 0; j < FN; ++j) bf[j] = *reinterpret_cast<const v4i*>(&lb[(wn0 + j * MS + lr) * RS + kk + lk]);
-            v4i a_cur = *reinterpret_cast<const v4i*>(&la[(wm0 + lr) * RS + kk + lk]);
+            for (int j = 0; j < FN; ++j) bf[j] = *reinterpret_cast<const v4i*>(&lb[at(wn0 + j * MS + lr, kk + lk)]);
+            v4i a_cur = *reinterpret_cast<const v4i*>(&la[at(wm0 + lr, kk + lk)]);
 #pragma unroll
             for (int i = 0; i < FM; ++i) {
                 v4i a_next = a_cur;
-                if (i + 1 < FM) a_next = *reinterpret_cast<const v4i*>(&la[(wm0 + (i + 1) * MS + lr) * RS + kk + lk]);
+                if (i + 1 < FM) a_next = *reinterpret_cast<const v4i*>(&la[at(wm0 + (i + 1) * MS + lr, kk + lk)]);
                 __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
                 for (int j = 0; j < FN; ++j) acc[i][j] = MF::mma(a_cur, bf[j], acc[i][j]);
@@ -955,6 +988,9 @@ struct CrtArgs {
 constexpr int kCrtRows = ZG_CRT_ROWS;  // 4 or 8
 static_assert(kCrtRows == 4 || kCrtRows == 8, "rows per reconstruction thread");
 constexpr int kCrtThreads = 256;
+// ND: number of 40-bit digits (cc.ndig), a template constant so the digit
+// loops carry no per-digit selects.
+template <int ND>
 __global__ __launch_bounds__(kCrtThreads) void crt_reconstruct_kernel(CrtArgs p) {
     __shared__ double prods[kCrtThreads * (kCrtRows + 1)];  // row stride 9: conflict-free 8-byte writes
     const int base = blockIdx.x * kCrtThreads * kCrtRows;
@@ -963,7 +999,7 @@ __global__ __launch_bounds__(kCrtThreads) void crt_reconstruct_kernel(CrtArgs p)
     const int j = blockIdx.y;
     const int sb = p.sig_b[j];
     if (i0 < p.m) {
-        const int nd = p.cc.ndig, nm = p.cc.nmod;
+        const int nm = p.cc.nmod;
         const uint8_t* u = p.u + (int64_t)j * p.ldu + i0;
         uint32_t w[kMaxModuli][kCrtRows / 4];
 #pragma unroll
@@ -978,7 +1014,7 @@ __global__ __launch_bounds__(kCrtThreads) void crt_reconstruct_kernel(CrtArgs p)
                 }
             }
         }
-        double S[kCrtRows][kMaxDigits] = {};
+        double S[kCrtRows][ND] = {};
 #pragma unroll
         for (int l = 0; l < kMaxModuli; ++l) {
             if (l < nm) {
@@ -987,8 +1023,7 @@ __global__ __launch_bounds__(kCrtThreads) void crt_reconstruct_kernel(CrtArgs p)
                     const uint32_t word = w[l][r >> 2];
                     const double ul = (double)((word >> (8 * (r & 3))) & 0xffu);
 #pragma unroll
-                    for (int d = 0; d < kMaxDigits; ++d)
-                        if (d < nd) S[r][d] = fma(p.cc.h[l][d], ul, S[r][d]);  // exact: integer < 2^53
+                    for (int d = 0; d < ND; ++d) S[r][d] = fma(p.cc.h[l][d], ul, S[r][d]);  // exact: integer < 2^53
                 }
             }
         }
@@ -997,28 +1032,24 @@ __global__ __launch_bounds__(kCrtThreads) void crt_reconstruct_kernel(CrtArgs p)
             const int i = i0 + r;
             double xq = 0.0;
 #pragma unroll
-            for (int d = 0; d < kMaxDigits; ++d)
-                if (d < nd) xq = fma(S[r][d], p.cc.inv_p_dig[d], xq);
+            for (int d = 0; d < ND; ++d) xq = fma(S[r][d], p.cc.inv_p_dig[d], xq);
             const double q = rint(xq);
             // C' = sum_d D_d 2^(40 (nd-1-d)) with D_d = S_d - q P_d (exact, |D_d| < 2^53).
             // Normalize to balanced 40-bit digits with exact carries, so that every
             // lower part is below half an ulp of what is added on top of it; the sum
             // is then within one rounding of C' and exact when C' is representable.
-            double D[kMaxDigits];
+            double D[ND];
 #pragma unroll
-            for (int d = 0; d < kMaxDigits; ++d) D[d] = (d < nd) ? fma(-q, p.cc.p[d], S[r][d]) : 0.0;
+            for (int d = 0; d < ND; ++d) D[d] = fma(-q, p.cc.p[d], S[r][d]);
 #pragma unroll
-            for (int d = kMaxDigits - 1; d >= 1; --d) {
-                if (d < nd) {
-                    const double carry = rint(D[d] * 0x1p-40);
-                    D[d] = fma(-carry, 0x1p40, D[d]);  // exact, |D_d| <= 2^39
-                    D[d - 1] += carry;                 // exact, integers well below 2^53
-                }
+            for (int d = ND - 1; d >= 1; --d) {
+                const double carry = rint(D[d] * 0x1p-40);
+                D[d] = fma(-carry, 0x1p40, D[d]);  // exact, |D_d| <= 2^39
+                D[d - 1] += carry;                 // exact, integers well below 2^53
             }
             double cp = 0.0;
 #pragma unroll
-            for (int d = kMaxDigits - 1; d >= 0; --d)
-                if (d < nd) cp = ldexp(D[d], p.cc.dig_exp[d]) + cp;
+            for (int d = ND - 1; d >= 0; --d) cp = ldexp(D[d], p.cc.dig_exp[d]) + cp;
             const int sa = (i < p.m) ? p.sig_a[i] : 0;
             prods[t * (kCrtRows + 1) + r] = (sa == kNonFinite || sb == kNonFinite)
                                                 ? __longlong_as_double(0x7ff8000000000000ll)
@@ -1098,13 +1129,13 @@ hipError_t planes(const double* x, int64_t ld, int rows, int k, int kp, bool kco
 }
 }  // namespace
 
-template <int BM, int BN, int BK, int WM, int WN, int OCC, class MF = I8Mfma<16>>
+template <int BM, int BN, int BK, int WM, int WN, int OCC, class MF = I8Mfma<16>, bool SWZ = false>
 static hipError_t i8_launch_cfg(I8Args p, int nout, hipStream_t s) {
     p.tiles_m = (p.m + BM - 1) / BM;
     p.tiles_n = (p.n + BN - 1) / BN;
     if (p.kp % BK != 0) return hipErrorInvalidValue;
-    const size_t lds = 2 * (size_t)(BM + BN) * (BK + ZG_I8_LDS_PAD);
-    auto kern = i8_gemm_kernel<BM, BN, BK, WM, WN, OCC, MF>;
+    const size_t lds = 2 * (size_t)(BM + BN) * (SWZ ? BK : BK + ZG_I8_LDS_PAD);
+    auto kern = i8_gemm_kernel<BM, BN, BK, WM, WN, OCC, MF, SWZ>;
     hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
                                        (int)lds);
     if (e != hipSuccess) return e;
@@ -1146,6 +1177,21 @@ hipError_t i8_gemm_launch(const I8Args& args, int nout, hipStream_t s, int tile)
         case 11: return i8_dma_launch_cfg<256, 256, 4, 2, 3, 2, I8Mfma<32>>(args, nout, s);
         case 12: return i8_dma_launch_cfg<256, 128, 4, 2, 4, 2, I8Mfma<32>>(args, nout, s);
         case 13: return i8_dma_launch_cfg<256, 256, 2, 2, 4, 1, I8Mfma<32>>(args, nout, s);
+        // unpadded LDS rows with the XOR chunk swizzle
+        case 14: return i8_launch_cfg<256, 256, 128, 4, 2, 2, I8Mfma<16>, true>(args, nout, s);
+        case 15: return i8_launch_cfg<256, 256, 128, 2, 2, 1, I8Mfma<16>, true>(args, nout, s);
+        case 16: return i8_launch_cfg<256, 256, 128, 4, 2, 2, I8Mfma<32>, true>(args, nout, s);
+        case 17: return i8_launch_cfg<128, 128, 128, 2, 2, 2, I8Mfma<16>, true>(args, nout, s);
+        // 16 waves of 64 x 64 (4 per SIMD: twice the latency hiding of the 8-wave tiles)
+        case 18: return i8_launch_cfg<256, 256, 64, 4, 4, 4, I8Mfma<16>>(args, nout, s);
+        case 19: return i8_launch_cfg<256, 256, 128, 4, 4, 4, I8Mfma<16>, true>(args, nout, s);
+        case 20: return i8_launch_cfg<256, 256, 64, 4, 4, 4, I8Mfma<32>>(args, nout, s);
+        case 21: return i8_launch_cfg<256, 256, 128, 4, 4, 4, I8Mfma<32>, true>(args, nout, s);
+        // 8 waves of 64 x 64, two workgroups per CU (independent barriers)
+        case 22: return i8_launch_cfg<128, 256, 64, 2, 4, 4, I8Mfma<16>>(args, nout, s);
+        case 23: return i8_launch_cfg<256, 128, 64, 4, 2, 4, I8Mfma<16>>(args, nout, s);
+        case 24: return i8_launch_cfg<128, 256, 128, 2, 4, 4, I8Mfma<16>, true>(args, nout, s);
+        case 25: return i8_launch_cfg<256, 256, 128, 4, 4, 4, I8Mfma<16>>(args, nout, s);
         default: return hipErrorInvalidValue;
     }
 }
@@ -1260,7 +1306,12 @@ int ozaki_dgemm(bool ta, bool tb, int m, int n, int k, double alpha, const doubl
             ca.sig_a = sig_a; ca.sig_b = sig_b; ca.alpha = alpha; ca.beta = beta; ca.c = c; ca.ldc = ldc;
             ca.cc = cc;
             dim3 grid((m + kCrtThreads * kCrtRows - 1) / (kCrtThreads * kCrtRows), n);
-            crt_reconstruct_kernel<<<grid, kCrtThreads, 0, s>>>(ca);
+            switch (cc.ndig) {
+                case 2: crt_reconstruct_kernel<2><<<grid, kCrtThreads, 0, s>>>(ca); break;
+                case 3: crt_reconstruct_kernel<3><<<grid, kCrtThreads, 0, s>>>(ca); break;
+                case 4: crt_reconstruct_kernel<4><<<grid, kCrtThreads, 0, s>>>(ca); break;
+                default: crt_reconstruct_kernel<5><<<grid, kCrtThreads, 0, s>>>(ca); break;
+            }
             e = hipGetLastError();
         }
     } else if (e == hipSuccess) {

@@ -62,8 +62,9 @@ struct I8Args {
 };
 
 // tile configurations 0..kI8Tiles-1 of i8_gemm_launch (0..5: 16x16x64 MFMA, 6..9: 32x32x32 MFMA,
-// 10..13: 32x32x32 MFMA fed by direct global -> LDS loads)
-constexpr int kI8Tiles = 14;
+// 10..13: 32x32x32 MFMA fed by direct global -> LDS loads, 14..17: unpadded XOR-swizzled LDS rows,
+// 18..21 and 25: 16 waves of 64x64, 22..24: 8 waves of 64x64, two workgroups per CU)
+constexpr int kI8Tiles = 26;
 hipError_t i8_gemm_launch(const I8Args& args, int nout, hipStream_t s, int tile);
 // tile configurations 0..kBf16Tiles-1 of the BF16 slice GEMM (same I8Args, bf16 planes, kp in bytes)
 constexpr int kBf16Tiles = 4;
