@@ -310,3 +310,106 @@ def test_splitmix_generator_matches_numpy(oracle):
     p = synth.uniform(12345, 1000, 5000)
     assert np.array_equal(c, p)
     assert p.min() >= -1.0 and p.max() < 1.0
+
+
+# ---------------------------------------------------------------------------
+# FP32 (sgemm) pins: the same restatement in single precision
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("case", all_cases())
+def test_sgemm_integer_inputs_bitwise_exact(oracle, case):
+    """Integer inputs in [-14, 14] whose sums stay below 2^24 (every order
+    exact in FP32): the oracle's SGEMM equals exact arithmetic bitwise and
+    leaves the padding (sentinel -777) untouched."""
+    ta, tb, m, n, k, lda, ldb, ldc, al, be = case
+    if 14 * 14 * k + 2 * 14 >= 2**24 / 8:
+        pytest.skip("sums may exceed 2^24 in FP32")
+    if al == 0.001:
+        al = 0.125
+    a, b, c = operands(ta, tb, m, n, k, lda, ldb, ldc, np.float32, True)
+    want = int_gemm(ta, tb, m, n, k, al, a, lda, b, ldb, be, c, ldc)
+    got = c.copy()
+    oracle.sgemm(ta, tb, m, n, k, al, a, lda, b, ldb, be, got, ldc)
+    assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+
+
+def test_sgemm_fma_chain_order_pinned(oracle):
+    """packed_simd.zig:140-162 in FP32: a = [1, 1 + 2^-13], b = [-1, 1 - 2^-13]:
+    the fused chain gives fma(1 + 2^-13, 1 - 2^-13, -1) = -2^-26 exactly, while
+    the separately rounded product is 1 (the float below 1 is 1 - 2^-24), so
+    the small TN portable loop (gemm.zig:1422-1429) gives 0."""
+    e = np.float32(2.0**-13)
+    a = np.array([1.0, 1.0 + e], dtype=np.float32)
+    b = np.array([-1.0, 1.0 - e], dtype=np.float32)
+    c = np.zeros(1, dtype=np.float32)
+    oracle.sgemm("N", "N", 1, 1, 2, 1.0, a, 1, b, 2, 0.0, c, 1)
+    assert c[0] == np.float32(-(2.0**-26))
+    c2 = np.zeros(1, dtype=np.float32)
+    oracle.sgemm("T", "N", 1, 1, 2, 1.0, a, 2, b, 2, 0.0, c2, 1)
+    assert c2[0] == 0.0
+
+
+def test_sgemm_epilogue_rounding_pinned(oracle):
+    """epilogue.zig:10-39 in FP32: for beta not 0 or 1 the result is
+    fma(acc, alpha, RN(beta C)).  acc = alpha = 1 + 2^-23, C = 1,
+    beta = -(1 + 2^-22): alpha acc = 1 + 2^-22 + 2^-46 exactly, so the fused
+    form gives 2^-46 and a separately rounded alpha acc gives 0."""
+    u = 2.0**-23
+    a = np.array([1.0 + u], dtype=np.float32)
+    b = np.array([1.0], dtype=np.float32)
+    c = np.array([1.0], dtype=np.float32)
+    got = c.copy()
+    oracle.sgemm("N", "N", 1, 1, 1, 1.0 + u, a, 1, b, 1, -(1.0 + 2 * u), got, 1)
+    assert got[0] == np.float32(2.0**-46)
+
+
+def test_sgemm_simd_tiers_and_threads_agree(oracle):
+    """AVX-512, AVX2 and scalar FP32 tiles, and any thread count, give the same bits."""
+    rng = np.random.default_rng(17)
+    m, n, k = 70, 45, 333
+    a = rng.uniform(-1, 1, m * k).astype(np.float32)
+    b = rng.uniform(-1, 1, k * n).astype(np.float32)
+    c0 = rng.uniform(-1, 1, m * n).astype(np.float32)
+    native = oracle.isa()
+    outs = []
+    for isa in (2, 1, 0):
+        if isa > native:
+            continue
+        oracle.force_isa(isa)
+        c = c0.copy()
+        oracle.sgemm("N", "N", m, n, k, 1.5, a, m, b, k, 0.5, c, m)
+        outs.append(c)
+    oracle.force_isa(-1)
+    for t in (1, 3):
+        oracle.set_threads(t)
+        c = c0.copy()
+        oracle.sgemm("N", "N", m, n, k, 1.5, a, m, b, k, 0.5, c, m)
+        outs.append(c)
+    oracle.set_threads(0)
+    assert len(outs) >= 3
+    for o in outs[1:]:
+        assert np.array_equal(o, outs[0])
+
+
+@pytest.mark.parametrize("ta,tb", [("N", "N"), ("T", "T")])
+def test_sgemm_random_within_bound_of_float64(oracle, ta, tb):
+    """|C32 - C_exact| <= 2 (K + 2) u32 (|alpha||A||B| + |beta||C0|) against the
+    float64 product of the same float32 inputs (exact to far below u32)."""
+    rng = np.random.default_rng(23)
+    m, n, k = 90, 70, 500
+    lda = (m if ta == "N" else k) + 2
+    ldb = (k if tb == "N" else n) + 3
+    a = rng.uniform(-1, 1, lda * (k if ta == "N" else m)).astype(np.float32)
+    b = rng.uniform(-1, 1, ldb * (n if tb == "N" else k)).astype(np.float32)
+    c = rng.uniform(-1, 1, m * n).astype(np.float32)
+    got = c.copy()
+    oracle.sgemm(ta, tb, m, n, k, 1.25, a, lda, b, ldb, -0.5, got, m)
+
+    def mat(x, rows, cols, ld):
+        return x[: ld * cols].reshape(cols, ld)[:, :rows].T.astype(np.float64)
+
+    A = mat(a, m, k, lda) if ta == "N" else mat(a, k, m, lda).T
+    B = mat(b, k, n, ldb) if tb == "N" else mat(b, n, k, ldb).T
+    C0 = mat(c, m, n, m)
+    exact = 1.25 * (A @ B) - 0.5 * C0
+    bound = 2 * (k + 2) * 2.0**-24 * (1.25 * np.abs(A) @ np.abs(B) + 0.5 * np.abs(C0))
+    assert np.all(np.abs(mat(got, m, n, m) - exact) <= bound)
