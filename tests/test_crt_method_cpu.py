@@ -94,20 +94,32 @@ def fma_exact(a, b, c):
     return float(Fraction(a) * Fraction(b) + Fraction(c))
 
 
-def sym_mod(v, m, two_step=True):
-    """The residue extraction of the plane kernel, in IEEE doubles: one exact
-    reduction step, a second one when |r| > m, then the symmetric fix-up."""
-    inv_m = 1.0 / m
-    r = fma_exact(-float(round(v * inv_m)), float(m), v)
-    if two_step and abs(r) > m:
-        r = fma_exact(-float(round(r * inv_m)), float(m), r)
-    ri = int(r)
-    half = m >> 1
-    if ri >= half + (m & 1):
-        ri -= m
-    if ri < -half:
-        ri += m
-    return ri
+def exact_double(x):
+    """An integer the kernel computes with one exact FMA: it must be a double."""
+    assert float(x) == x, x
+    return int(x)
+
+
+def split_residue(v, m, parts):
+    """The residue extraction of the plane kernel, in IEEE doubles: a = v is
+    split once into a2 2^64 + a1 2^32 + a0 (truncated quotients, exact
+    differences; a2 = 0 for two parts), then t = a2 (2^64 mod m) +
+    a1 (2^32 mod m) + a0 and one rounded quotient for an odd m, a0 mod 256 as
+    a signed byte for m = 256."""
+    v = float(v)
+    a2 = math.trunc(v * 2.0**-64) if parts == 3 else 0
+    rest = exact_double(int(v) - a2 * 2**64)
+    a1 = math.trunc(float(rest) * 2.0**-32)
+    a0 = exact_double(rest - a1 * 2**32)
+    if m == 256:
+        u = exact_double(a0 - 256 * math.floor(float(a0) * 2.0**-8))
+        assert 0 <= u < 256
+        return u - 256 if u > 127 else u
+    t = exact_double(a1 * (2**32 % m) + a0)
+    if parts == 3:
+        t = exact_double(a2 * (2**64 % m) + t)
+    q = round(float(t) * (1.0 / m))  # rint: round half to even, as Python's round on floats
+    return t - m * q
 
 
 def exact_sym_mod(v, m):
@@ -116,33 +128,41 @@ def exact_sym_mod(v, m):
     return r - m if r >= half + (m & 1) else r
 
 
+def parts_for(nmod):
+    P = 1
+    for m in MODULI[:nmod]:
+        P *= m
+    alpha = (P.bit_length() - 2) // 2
+    return alpha, (3 if alpha + 1 >= 63 else 2)
+
+
 def test_residue_extraction_exact_for_every_modulus_count():
     """Scaled inputs are integers with |a'| <= 2^(alpha+1); for every modulus
-    count the kernel's reduction returns the exact symmetric residue (the
-    largest alpha, 20 moduli, needs the second step)."""
+    count the kernel's split reduction returns the exact balanced residue
+    (three parts from 16 moduli on, where |a'| passes 2^63)."""
     rng = random.Random(11)
     for nmod in (8, 14, 16, 20):
-        P = 1
-        for m in MODULI[:nmod]:
-            P *= m
-        alpha = (P.bit_length() - 2) // 2
+        alpha, parts = parts_for(nmod)
         for _ in range(400):
             e = rng.randint(0, alpha + 1)
             v = float(rng.randint(-2**52, 2**52)) * 2.0 ** max(0, e - 52)
             v = float(round(v))
             for m in MODULI[:nmod]:
-                assert sym_mod(v, m) == exact_sym_mod(v, m), (nmod, v, m)
+                assert split_residue(v, m, parts) == exact_sym_mod(v, m), (nmod, v, m)
 
 
-def test_single_reduction_step_is_not_enough_at_20_moduli():
-    """Planted mistake: dropping the conditional second step gives wrong
-    residues for the largest scaled inputs (|a'| near 2^79)."""
+def test_two_parts_are_not_enough_at_20_moduli():
+    """Planted mistake: the two-part split for the largest scaled inputs
+    (|a'| near 2^79) leaves t far above 2^53, so its sums are not exact and
+    the residues are wrong."""
     rng = random.Random(12)
     bad = 0
     for _ in range(200):
         v = float(rng.randint(2**52, 2**53 - 1)) * 2.0**26
-        for m in MODULI[:20]:
-            if sym_mod(v, m, two_step=False) != exact_sym_mod(v, m):
+        for m in MODULI[1:20]:
+            try:
+                bad += split_residue(v, m, 2) != exact_sym_mod(v, m)
+            except AssertionError:
                 bad += 1
     assert bad > 0
 
@@ -152,42 +172,25 @@ def balanced(v, m):
     return r - m if r > (m - 1) // 2 else r
 
 
-def test_two_step_odd_residue_is_exact():
-    """The plane kernel's residue of an integer-valued double v for an odd
-    modulus m: t = v - m rint(v inv_m), then r = t - m rint(t inv_m), both
-    differences exact (fma of integers with an integer result).  Emulated with
-    IEEE doubles for the products and roundings and exact integers for the
-    differences, r is the balanced residue in [-(m-1)/2, (m-1)/2] for every v up
-    to 2^100 in magnitude, including values next to multiples of m and to
-    half-multiples, where a one-step rounding of v / m can go either way."""
+def test_split_residue_exact_next_to_multiples():
+    """The split reduction at the values where a rounded quotient can go
+    either way: next to multiples and to half-multiples of every odd modulus,
+    at every magnitude up to 2^79 and with either part count that covers it."""
     rnd = random.Random(17)
     cases = []
     for m in MODULI[1:]:  # the odd moduli
-        inv = 1.0 / m
-        for bits in (8, 30, 53, 56, 78, 100):
-            for _ in range(400):
-                v = rnd.randrange(-(1 << bits), 1 << bits)
-                if bits > 53:
-                    v = int(float(v))  # an integer-valued double
-                cases.append((v, m, inv))
-            for k in (1, 3, 12345, (1 << 40) + 7):
+        for bits in (8, 30, 53, 56, 62, 70, 79):
+            for _ in range(200):
+                v = int(float(rnd.randrange(-(1 << bits), 1 << bits)))
+                cases.append(v)
+            for k in (1, 3, 12345, (1 << 40) + 7, (1 << 60) // m):
                 for d in (-1, 0, 1):
-                    cases.append((k * m + d, m, inv))
-                    cases.append((-(k * m) + d, m, inv))
-                    cases.append((k * m + (m - 1) // 2 + d, m, inv))
-    for v, m, inv in cases:
-        t = v - m * int(round(float(v) * inv))  # rint = round half to even, as Python's round on floats
-        r = t - m * int(round(float(t) * inv))
-        assert r == balanced(v, m), (v, m, t, r)
+                    for v in (k * m + d, -(k * m) + d, k * m + (m - 1) // 2 + d):
+                        if float(v) == v:
+                            cases.append(v)
+    for v in cases:
+        for m in MODULI:
+            for parts in ((2, 3) if abs(v) < 2**62 else (3,)):
+                assert split_residue(v, m, parts) == balanced(v, m), (v, m, parts)
 
 
-def test_one_step_residue_needs_the_second_step():
-    """A plausible shortcut, one rounded quotient, leaves |r| above (m-1)/2 for large v."""
-    rnd = random.Random(3)
-    m, inv = 173, 1.0 / 173
-    bad = 0
-    for _ in range(2000):
-        v = int(float(rnd.randrange(-(1 << 78), 1 << 78)))
-        r = v - m * int(round(float(v) * inv))
-        bad += abs(r) > (m - 1) // 2
-    assert bad > 0

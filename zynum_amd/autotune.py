@@ -37,7 +37,7 @@ from . import _lib, blas, grouped, synth
 CSRC = Path(__file__).resolve().parent / "csrc"
 PLAIN_TILES = 12         # kTileConfigs
 GROUPED_TILES = 3        # kGroupedCfgs
-I8_TILES = 14            # kI8Tiles
+I8_TILES = 26            # kI8Tiles
 PLAIN_SIZES = (8192, 2048, 1024)
 PLAIN_CLASS_MIN_TILES = {8192: 512, 2048: 128, 1024: 0}  # 128x128 tiles: 4096, 256, 64 at these sizes
 GROUPED_BUCKETS = ((16, 48), (48, 96), (96, 192), (192, 512))

@@ -282,33 +282,9 @@ struct PlaneArgs {
     int bf16;              // slices stored as bf16 (2 bytes, exact: |s| <= 64) instead of int8
     int m[kMaxModuli];
     double inv_m[kMaxModuli];
+    double c32[kMaxModuli];  // 2^32 mod m
+    double c64[kMaxModuli];  // 2^64 mod m
 };
-
-__device__ __forceinline__ int sym_mod(double v, int m, double inv_m) {
-    // v is an integer-valued double (|v| < 2^100); returns v mod m in [-m/2, m/2).
-    // rint(v inv_m) differs from v/m by at most 1/2 + |v| 2^-52 / m, so
-    // r <- v - m rint(v inv_m) is an integer with |r| <= m/2 + |v| 2^-52 < 2^48
-    // and the fma computing it is exact.  When that leaves |r| > m (|v| above
-    // about 2^59) a second step leaves |r| <= m/2 + 1.
-    double r = fma(-rint(v * inv_m), (double)m, v);
-    if (fabs(r) > (double)m) r = fma(-rint(r * inv_m), (double)m, r);
-    int ri = (int)r;
-    const int half = m >> 1;
-    if (ri >= half + (m & 1)) ri -= m;
-    if (ri < -half) ri += m;
-    return ri;
-}
-
-// Balanced residue for an odd modulus, in two exact reduction steps and no
-// integer fix-ups: the first leaves an integer |t| <= m/2 + |v| 2^-52 < 2^48
-// (as sym_mod argues); the second rounds t / m correctly, because the
-// computed t inv_m is within |t / m| 2^-51 < 2^-10 of t / m while, m being
-// odd, t / m is at least 1 / (2m) >= 2^-9 away from every half-integer.  So
-// r = t - m rint(t / m) lies in [-(m-1)/2, (m-1)/2]: the value sym_mod returns.
-__device__ __forceinline__ int sym_mod_odd(double v, int m, double inv_m) {
-    const double t = fma(-rint(v * inv_m), (double)m, v);
-    return (int)fma(-rint(t * inv_m), (double)m, t);
-}
 
 // Rows of op(X) per block of the plane kernel; each of the 256 threads then
 // converts EPT = TR * 128 / 256 consecutive k of one row.
@@ -316,13 +292,29 @@ __device__ __forceinline__ int sym_mod_odd(double v, int m, double inv_m) {
 #define ZG_PLANE_ROWS 16
 #endif
 
-template <bool KCONTIG>
+// Column of element k of a plane tile row: one pad double after every 8, so
+// that the 8-element runs the threads read, and the columns the loads write,
+// fall in distinct LDS banks (a row of 128 + 16 + 1 doubles).
+__device__ __forceinline__ int plane_col(int kk) { return kk + (kk >> 3); }
+
+// Residues of the CRT form.  a = x 2^sigma rounded is an exact integer with
+// |a| <= 2^(alpha+1) < 2^80.  It is split once per element into parts of at
+// most 32 bits, a = a2 2^64 + a1 2^32 + a0, each part with the sign of what it
+// is taken from, so every difference is exact (PARTS = 2 when |a| < 2^63:
+// a2 = 0).  The residue of a modulo m is then that of
+// t = a2 (2^64 mod m) + a1 (2^32 mod m) + a0, an exact integer below 2^41.
+// For an odd m one rounded quotient gives the balanced residue
+// r = t - m rint(t inv_m) in [-(m-1)/2, (m-1)/2]: the computed t inv_m is
+// within 2^-11 of t / m, which, m being odd, is at least 1 / (2m) >= 2^-9 away
+// from every half-integer.  For m = 256 the residue is a0 mod 256 as a signed
+// byte (the other parts are multiples of 2^32).
+template <bool KCONTIG, int PARTS>
 __global__ __launch_bounds__(256) void plane_kernel(PlaneArgs p) {
     constexpr int TR = ZG_PLANE_ROWS, TK = 128;
     constexpr int EPT = TR * TK / 256;  // elements per thread (8 or 16)
     constexpr int TPR = TK / EPT;       // threads per row
     static_assert((EPT == 8 || EPT == 16) && TPR * EPT == TK, "plane tile: 16 or 32 rows per block");
-    __shared__ double tile[TR][TK + 1];
+    __shared__ double tile[TR][TK + TK / 8 + 1];
     const int r0 = blockIdx.x * TR, k0 = blockIdx.y * TK;
     const int t = threadIdx.x;
     // load: coalesced along the contiguous dimension
@@ -335,7 +327,7 @@ __global__ __launch_bounds__(256) void plane_kernel(PlaneArgs p) {
         const int gr = r0 + r, gk = k0 + kk;
         double v = 0.0;
         if (gr < p.rows && gk < p.k) v = KCONTIG ? p.x[gk + (int64_t)gr * p.ld] : p.x[gr + (int64_t)gk * p.ld];
-        tile[r][kk] = v;
+        tile[r][plane_col(kk)] = v;
     }
     __syncthreads();
     // each thread: one row, EPT consecutive k -> EPT (bf16: 2 EPT) bytes per plane
@@ -345,6 +337,7 @@ __global__ __launch_bounds__(256) void plane_kernel(PlaneArgs p) {
     const int sg = p.sigma[gr];
     const int eb = p.bf16 ? 2 : 1;
     int8_t* dst = p.out + ((int64_t)gr * p.kp + k0 + kq) * eb;
+    const double* src = &tile[r][plane_col(kq)];  // EPT contiguous values (kq is a multiple of 8)
     constexpr int W = EPT / 4;  // 32-bit words of int8 output per plane
     if (sg == kNonFinite) {
         // the row's results are set to NaN by the final kernel; its planes are zero
@@ -355,22 +348,47 @@ __global__ __launch_bounds__(256) void plane_kernel(PlaneArgs p) {
         return;
     }
     if (p.crt) {
-        double a[EPT];
+        double a2[EPT], a1[EPT], a0[EPT];
 #pragma unroll
-        for (int b = 0; b < EPT; ++b) a[b] = rint(ldexp(tile[r][kq + b], sg));  // exact integers, |a| <= 2^(alpha+1)
+        for (int b = 0; b < EPT; ++b) {
+            const double a = rint(ldexp(src[(b >> 3) * 9 + (b & 7)], sg));
+            double rest = a;
+            a2[b] = 0.0;
+            if constexpr (PARTS == 3) {
+                a2[b] = trunc(a * 0x1p-64);
+                rest = fma(-a2[b], 0x1p64, a);
+            }
+            a1[b] = trunc(rest * 0x1p-32);
+            a0[b] = fma(-a1[b], 0x1p32, rest);
+        }
         for (int l = 0; l < p.nplanes; ++l) {
             const int ml = p.m[l];
-            const double iml = p.inv_m[l];
             uint32_t pw[W];
+            if (ml == 256) {
 #pragma unroll
-            for (int w = 0; w < W; ++w) {
-                uint32_t word = 0;
+                for (int w = 0; w < W; ++w) {
+                    uint32_t word = 0;
 #pragma unroll
-                for (int b = 0; b < 4; ++b) {
-                    const int res = (ml & 1) ? sym_mod_odd(a[4 * w + b], ml, iml) : sym_mod(a[4 * w + b], ml, iml);
-                    word |= ((uint32_t)(uint8_t)(int8_t)res) << (8 * b);
+                    for (int b = 0; b < 4; ++b) {
+                        const double u = fma(-floor(a0[4 * w + b] * 0x1p-8), 256.0, a0[4 * w + b]);  // in [0, 256)
+                        word |= ((uint32_t)u & 0xffu) << (8 * b);
+                    }
+                    pw[w] = word;
                 }
-                pw[w] = word;
+            } else {
+                const double dm = (double)ml, iml = p.inv_m[l], c32 = p.c32[l], c64 = p.c64[l];
+#pragma unroll
+                for (int w = 0; w < W; ++w) {
+                    uint32_t word = 0;
+#pragma unroll
+                    for (int b = 0; b < 4; ++b) {
+                        double tt = fma(a1[4 * w + b], c32, a0[4 * w + b]);
+                        if constexpr (PARTS == 3) tt = fma(a2[4 * w + b], c64, tt);
+                        const int res = (int)fma(-rint(tt * iml), dm, tt);
+                        word |= ((uint32_t)(uint8_t)(int8_t)res) << (8 * b);
+                    }
+                    pw[w] = word;
+                }
             }
             uint32_t* d = reinterpret_cast<uint32_t*>(dst + (int64_t)l * p.plane_stride);
             if constexpr (W == 4) *reinterpret_cast<uint4*>(d) = uint4{pw[0], pw[1], pw[2], pw[3]};
@@ -380,7 +398,7 @@ __global__ __launch_bounds__(256) void plane_kernel(PlaneArgs p) {
         // slices: s_1 = rint(x 2^6), then 7 more bits per slice (|s_j| <= 64)
         double rem[EPT];
 #pragma unroll
-        for (int b = 0; b < EPT; ++b) rem[b] = ldexp(tile[r][kq + b], sg);  // |rem| < 1
+        for (int b = 0; b < EPT; ++b) rem[b] = ldexp(src[(b >> 3) * 9 + (b & 7)], sg);  // |rem| < 1
         for (int l = 0; l < p.nplanes; ++l) {
             const int shift = (l == 0) ? 6 : 6 + 7 * l;
             double v[EPT];
@@ -1120,11 +1138,26 @@ hipError_t planes(const double* x, int64_t ld, int rows, int k, int kp, bool kco
     pa.x = x; pa.ld = ld; pa.rows = rows; pa.k = k; pa.kp = kp; pa.sigma = sigma;
     pa.out = out; pa.plane_stride = plane_stride; pa.nplanes = nplanes; pa.crt = crt ? 1 : 0;
     pa.bf16 = bf16 ? 1 : 0;
-    if (crt)
-        for (int l = 0; l < nplanes; ++l) { pa.m[l] = cc->m[l]; pa.inv_m[l] = cc->inv_m[l]; }
+    if (crt) {
+        for (int l = 0; l < nplanes; ++l) {
+            const int m = cc->m[l];
+            pa.m[l] = m;
+            pa.inv_m[l] = cc->inv_m[l];
+            const uint64_t c32 = (1ull << 32) % (uint64_t)m;
+            pa.c32[l] = (double)c32;
+            pa.c64[l] = (double)((c32 * c32) % (uint64_t)m);
+        }
+    }
+    // |a| <= 2^(alpha+1) fits two parts below 2^63
+    const bool three = crt && cc->alpha + 1 >= 63;
     dim3 grid((rows + ZG_PLANE_ROWS - 1) / ZG_PLANE_ROWS, kp / 128);
-    if (kcontig) plane_kernel<true><<<grid, 256, 0, s>>>(pa);
-    else plane_kernel<false><<<grid, 256, 0, s>>>(pa);
+    if (kcontig) {
+        if (three) plane_kernel<true, 3><<<grid, 256, 0, s>>>(pa);
+        else plane_kernel<true, 2><<<grid, 256, 0, s>>>(pa);
+    } else {
+        if (three) plane_kernel<false, 3><<<grid, 256, 0, s>>>(pa);
+        else plane_kernel<false, 2><<<grid, 256, 0, s>>>(pa);
+    }
     return hipGetLastError();
 }
 }  // namespace
