@@ -142,10 +142,13 @@ bool crt_setup(int nmod, int64_t k, CrtConsts& cc) {
 // operand statistics: per-row (of op(X)) max |x| and scaled sum of squares
 // element (r, k) of op(X) is x[k + r*ld] when kcontig, x[r + k*ld] otherwise
 // ---------------------------------------------------------------------------
-__global__ void fill_u64_kernel(uint64_t* p, int64_t n, uint64_t v) {
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-        p[i] = v;
+__global__ void fill2_u64_kernel(uint64_t* p, int64_t n, uint64_t* q, int64_t nq, uint64_t v) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n + nq; i += (int64_t)gridDim.x * blockDim.x) {
+        if (i < n) p[i] = v;
+        else q[i - n] = v;
+    }
 }
+
 
 // One pass over op(X): per-row max |x| (atomicMax on the bit pattern, which
 // orders like the value for non-negative doubles and puts NaN above +inf) and
@@ -154,16 +157,37 @@ __global__ void fill_u64_kernel(uint64_t* p, int64_t n, uint64_t v) {
 // Strided rows: one thread per row and chunk.  Contiguous rows: one wave per
 // row, lanes over k.
 constexpr int kStatChunk = 256;
+// Smallest N of a CRT product that runs in parts on two streams, and the
+// number of parts.
+constexpr int kCrtSplitMinN = 4096;
+#ifndef ZG_CRT_PARTS
+#define ZG_CRT_PARTS 2
+#endif
+constexpr int kCrtParts = ZG_CRT_PARTS;
 // sigma of a row or column holding a NaN or an infinity: its results are NaN
 constexpr int kNonFinite = 0x7fffffff;
 
+// One operand of a statistics (or plane) launch: workgroups [first, first +
+// blocks) of the grid take its rows.
+struct StatOp {
+    const double* x;
+    int64_t ld;
+    int rows, kcontig;
+    uint64_t* maxbits;
+    double* part;
+    int blocks;
+};
+
 template <bool KCONTIG>
-__global__ __launch_bounds__(256) void row_stats_kernel(const double* __restrict__ x, int64_t ld, int rows, int k,
-                                                         uint64_t* __restrict__ maxbits, double* __restrict__ part) {
+__device__ __forceinline__ void row_stats_tile(const StatOp& o, int bx, int k) {
+    const double* __restrict__ x = o.x;
+    const int64_t ld = o.ld;
+    const int rows = o.rows;
+    uint64_t* __restrict__ maxbits = o.maxbits;
     const int k0 = blockIdx.y * kStatChunk, k1 = min(k, k0 + kStatChunk);
-    double* ps = part + (int64_t)blockIdx.y * rows;  // this chunk's partial sums
+    double* ps = o.part + (int64_t)blockIdx.y * rows;  // this chunk's partial sums
     if constexpr (!KCONTIG) {
-        const int r = blockIdx.x * 256 + threadIdx.x;
+        const int r = bx * 256 + threadIdx.x;
         if (r >= rows) return;
         double mx = 0.0, ss = 0.0;
         bool nan = false;  // fmax drops NaN operands: flag them separately
@@ -194,13 +218,13 @@ __global__ __launch_bounds__(256) void row_stats_kernel(const double* __restrict
             double v[kRowsTogether][kPer];
 #pragma unroll
             for (int q = 0; q < kRowsTogether; ++q) {
-                const int r = min(blockIdx.x * 64 + rr0 + q, rows - 1);
+                const int r = min(bx * 64 + rr0 + q, rows - 1);
 #pragma unroll
                 for (int t = 0; t < kPer; ++t) v[q][t] = x[min(k0 + lane + 64 * t, k1 - 1) + (int64_t)r * ld];
             }
 #pragma unroll
             for (int q = 0; q < kRowsTogether; ++q) {
-                const int r = blockIdx.x * 64 + rr0 + q;
+                const int r = bx * 64 + rr0 + q;
                 double mx = 0.0, ss = 0.0;
                 bool nan = false;
 #pragma unroll
@@ -227,6 +251,16 @@ __global__ __launch_bounds__(256) void row_stats_kernel(const double* __restrict
     }
 }
 
+// Both operands in one launch (their rows are independent): workgroups
+// [0, a.blocks) take op(A)'s rows, the rest op(B)'s.
+__global__ __launch_bounds__(256) void row_stats_kernel(StatOp a, StatOp b, int k) {
+    const bool first = (int)blockIdx.x < a.blocks;
+    const StatOp& o = first ? a : b;
+    const int bx = first ? (int)blockIdx.x : (int)blockIdx.x - a.blocks;
+    if (o.kcontig) row_stats_tile<true>(o, bx, k);
+    else row_stats_tile<false>(o, bx, k);
+}
+
 // Per-row scaling exponent sigma_r.
 //   CRT:    ||x_r 2^sigma||_2 <= 2^alpha.  The norm comes from the sum of
 //           squares when that is trustworthy (finite, and the max square is a
@@ -234,23 +268,35 @@ __global__ __launch_bounds__(256) void row_stats_kernel(const double* __restrict
 //           the bound sqrt(k) max |x|.
 //   slices: max |x_r 2^sigma| < 1
 //   rows holding a NaN or an infinity: kNonFinite
-__global__ void row_exponent_kernel(int rows, int k, const uint64_t* __restrict__ maxbits,
-                                    const double* __restrict__ part, int nchunks, int alpha, int crt,
-                                    int* __restrict__ sigma) {
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= rows) return;
-    const double mx = __longlong_as_double((long long)maxbits[r]);
+// Rows [0, a.rows) of the launch are op(A)'s, the next b.rows op(B)'s.
+struct ExpOp {
+    int rows;
+    const uint64_t* maxbits;
+    const double* part;
+    int* sigma;
+};
+
+__global__ void row_exponent_kernel(ExpOp a, ExpOp b, int k, int nchunks, int alpha, int crt) {
+    int r = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool first = r < a.rows;
+    const ExpOp& o = first ? a : b;
+    if (!first) r -= a.rows;
+    if (r >= o.rows) return;
+    const double mx = __longlong_as_double((long long)o.maxbits[r]);
     if (!isfinite(mx)) {
-        sigma[r] = kNonFinite;
+        o.sigma[r] = kNonFinite;
         return;
     }
     if (mx == 0.0) {
-        sigma[r] = 0;
+        o.sigma[r] = 0;
         return;
     }
     if (crt) {
+        // chunk sums added in chunk order (fixed, so the scaling is
+        // reproducible); the loads are issued ahead of the additions
         double ss = 0.0;
-        for (int c = 0; c < nchunks; ++c) ss += part[(int64_t)c * rows + r];
+#pragma unroll 8
+        for (int c = 0; c < nchunks; ++c) ss += o.part[(int64_t)c * o.rows + r];
         const double m2 = mx * mx;
         const bool ok = isfinite(ss) && isfinite(m2) && m2 >= 0x1p-1000 && ss >= m2;
         // relative rounding of a k-term sum is below k 2^-53 <= 2^-36 for k <= 2^17
@@ -258,9 +304,9 @@ __global__ void row_exponent_kernel(int rows, int k, const uint64_t* __restrict_
         int ex;
         const double fr = frexp(nrm, &ex);  // nrm = fr 2^ex, fr in [0.5, 1)
         (void)fr;
-        sigma[r] = alpha - ex;  // ||x|| 2^sigma < 2^ex 2^(alpha - ex) = 2^alpha
+        o.sigma[r] = alpha - ex;  // ||x|| 2^sigma < 2^ex 2^(alpha - ex) = 2^alpha
     } else {
-        sigma[r] = -(ilogb(mx) + 1);  // |x| 2^sigma < 1
+        o.sigma[r] = -(ilogb(mx) + 1);  // |x| 2^sigma < 1
     }
 }
 
@@ -270,15 +316,23 @@ __global__ void row_exponent_kernel(int rows, int k, const uint64_t* __restrict_
 // the FP64 reads are coalesced whatever the transpose and every plane row
 // segment written is 128 contiguous int8 (256 bf16) bytes.
 // ---------------------------------------------------------------------------
-struct PlaneArgs {
+// One operand of a plane launch: workgroups [first, first + blocks) along x
+// take its rows.
+struct PlaneOp {
     const double* x;
     int64_t ld;
-    int rows, k, kp;
+    int rows, kcontig;
     const int* sigma;
     int8_t* out;
     int64_t plane_stride;  // bytes between planes
+    int blocks;
+};
+
+struct PlaneArgs {
+    PlaneOp op[2];         // op(A), op(B)
+    int k, kp;
     int nplanes;
-    int crt;               // 1: residues modulo cc.m[l]; 0: 7-bit slices
+    int crt;               // 1: residues modulo m[l]; 0: 7-bit slices
     int bf16;              // slices stored as bf16 (2 bytes, exact: |s| <= 64) instead of int8
     int m[kMaxModuli];
     double inv_m[kMaxModuli];
@@ -308,14 +362,15 @@ __device__ __forceinline__ int plane_col(int kk) { return kk + (kk >> 3); }
 // within 2^-11 of t / m, which, m being odd, is at least 1 / (2m) >= 2^-9 away
 // from every half-integer.  For m = 256 the residue is a0 mod 256 as a signed
 // byte (the other parts are multiples of 2^32).
+constexpr int kPlaneTK = 128;
 template <bool KCONTIG, int PARTS>
-__global__ __launch_bounds__(256) void plane_kernel(PlaneArgs p) {
-    constexpr int TR = ZG_PLANE_ROWS, TK = 128;
+__device__ __forceinline__ void plane_tile(const PlaneArgs& p, const PlaneOp& o, int bx,
+                                           double (&tile)[ZG_PLANE_ROWS][kPlaneTK + kPlaneTK / 8 + 1]) {
+    constexpr int TR = ZG_PLANE_ROWS, TK = kPlaneTK;
     constexpr int EPT = TR * TK / 256;  // elements per thread (8 or 16)
     constexpr int TPR = TK / EPT;       // threads per row
     static_assert((EPT == 8 || EPT == 16) && TPR * EPT == TK, "plane tile: 16 or 32 rows per block");
-    __shared__ double tile[TR][TK + TK / 8 + 1];
-    const int r0 = blockIdx.x * TR, k0 = blockIdx.y * TK;
+    const int r0 = bx * TR, k0 = blockIdx.y * TK;
     const int t = threadIdx.x;
     // load: coalesced along the contiguous dimension
 #pragma unroll
@@ -326,23 +381,23 @@ __global__ __launch_bounds__(256) void plane_kernel(PlaneArgs p) {
         else { r = e & (TR - 1); kk = e / TR; }
         const int gr = r0 + r, gk = k0 + kk;
         double v = 0.0;
-        if (gr < p.rows && gk < p.k) v = KCONTIG ? p.x[gk + (int64_t)gr * p.ld] : p.x[gr + (int64_t)gk * p.ld];
+        if (gr < o.rows && gk < p.k) v = KCONTIG ? o.x[gk + (int64_t)gr * o.ld] : o.x[gr + (int64_t)gk * o.ld];
         tile[r][plane_col(kk)] = v;
     }
     __syncthreads();
     // each thread: one row, EPT consecutive k -> EPT (bf16: 2 EPT) bytes per plane
     const int r = t / TPR, kq = (t % TPR) * EPT;
     const int gr = r0 + r;
-    if (gr >= p.rows || k0 + kq >= p.kp) return;
-    const int sg = p.sigma[gr];
+    if (gr >= o.rows || k0 + kq >= p.kp) return;
+    const int sg = o.sigma[gr];
     const int eb = p.bf16 ? 2 : 1;
-    int8_t* dst = p.out + ((int64_t)gr * p.kp + k0 + kq) * eb;
+    int8_t* dst = o.out + ((int64_t)gr * p.kp + k0 + kq) * eb;
     const double* src = &tile[r][plane_col(kq)];  // EPT contiguous values (kq is a multiple of 8)
     constexpr int W = EPT / 4;  // 32-bit words of int8 output per plane
     if (sg == kNonFinite) {
         // the row's results are set to NaN by the final kernel; its planes are zero
         for (int l = 0; l < p.nplanes; ++l) {
-            uint32_t* d = reinterpret_cast<uint32_t*>(dst + (int64_t)l * p.plane_stride);
+            uint32_t* d = reinterpret_cast<uint32_t*>(dst + (int64_t)l * o.plane_stride);
             for (int w = 0; w < W * eb; ++w) d[w] = 0u;
         }
         return;
@@ -390,7 +445,7 @@ __global__ __launch_bounds__(256) void plane_kernel(PlaneArgs p) {
                     pw[w] = word;
                 }
             }
-            uint32_t* d = reinterpret_cast<uint32_t*>(dst + (int64_t)l * p.plane_stride);
+            uint32_t* d = reinterpret_cast<uint32_t*>(dst + (int64_t)l * o.plane_stride);
             if constexpr (W == 4) *reinterpret_cast<uint4*>(d) = uint4{pw[0], pw[1], pw[2], pw[3]};
             else *reinterpret_cast<uint2*>(d) = uint2{pw[0], pw[1]};
         }
@@ -407,7 +462,7 @@ __global__ __launch_bounds__(256) void plane_kernel(PlaneArgs p) {
                 v[b] = rint(ldexp(rem[b], shift));
                 rem[b] = fma(-v[b], ldexp(1.0, -shift), rem[b]);  // exact
             }
-            uint32_t* d = reinterpret_cast<uint32_t*>(dst + (int64_t)l * p.plane_stride);
+            uint32_t* d = reinterpret_cast<uint32_t*>(dst + (int64_t)l * o.plane_stride);
             if (p.bf16) {
                 // |v| <= 64: the float has at most 7 significant bits, so its top 16 bits are the bf16
 #pragma unroll
@@ -427,6 +482,18 @@ __global__ __launch_bounds__(256) void plane_kernel(PlaneArgs p) {
             }
         }
     }
+}
+
+// Both operands in one launch: workgroups [0, op[0].blocks) along x take
+// op(A)'s rows, the rest op(B)'s.
+template <int PARTS>
+__global__ __launch_bounds__(256) void plane_kernel(PlaneArgs p) {
+    __shared__ double tile[ZG_PLANE_ROWS][kPlaneTK + kPlaneTK / 8 + 1];
+    const bool first = (int)blockIdx.x < p.op[0].blocks;
+    const PlaneOp& o = first ? p.op[0] : p.op[1];
+    const int bx = first ? (int)blockIdx.x : (int)blockIdx.x - p.op[0].blocks;
+    if (o.kcontig) plane_tile<true, PARTS>(p, o, bx, tile);
+    else plane_tile<false, PARTS>(p, o, bx, tile);
 }
 
 // ---------------------------------------------------------------------------
@@ -1116,48 +1183,67 @@ struct Workspace {
     }
 };
 
-hipError_t operand_stats(const double* x, int64_t ld, int rows, int k, bool kcontig, uint64_t* maxbits,
-                         double* part, hipStream_t s) {
-    const int fb = std::max(1, std::min(1024, (rows + 255) / 256));
-    fill_u64_kernel<<<fb, 256, 0, s>>>(maxbits, rows, 0ull);
-    if (kcontig) {
-        dim3 grid((rows + 63) / 64, (k + kStatChunk - 1) / kStatChunk);
-        row_stats_kernel<true><<<grid, 256, 0, s>>>(x, ld, rows, k, maxbits, part);
-    } else {
-        dim3 grid((rows + 255) / 256, (k + kStatChunk - 1) / kStatChunk);
-        row_stats_kernel<false><<<grid, 256, 0, s>>>(x, ld, rows, k, maxbits, part);
-    }
+// Zeroes the row maxima of op(A) and op(B) (one launch).
+hipError_t stats_zero(uint64_t* max_a, int m, uint64_t* max_b, int n, hipStream_t s) {
+    const int fb = std::max(1, std::min(1024, (m + n + 255) / 256));
+    fill2_u64_kernel<<<fb, 256, 0, s>>>(max_a, m, max_b, n, 0ull);
     return hipGetLastError();
 }
 
-hipError_t planes(const double* x, int64_t ld, int rows, int k, int kp, bool kcontig, const int* sigma,
-                  int8_t* out, int64_t plane_stride, int nplanes, bool crt, const CrtConsts* cc, hipStream_t s,
-                  bool bf16 = false) {
-    PlaneArgs pa;
-    memset(&pa, 0, sizeof(pa));
-    pa.x = x; pa.ld = ld; pa.rows = rows; pa.k = k; pa.kp = kp; pa.sigma = sigma;
-    pa.out = out; pa.plane_stride = plane_stride; pa.nplanes = nplanes; pa.crt = crt ? 1 : 0;
-    pa.bf16 = bf16 ? 1 : 0;
+// Statistics and scaling exponents of rows of op(A) (m of them, contiguous in
+// k when kca) and of op(B) (its columns, contiguous in k when kcb), one
+// launch each over both; either count may be 0.  The chunk sums of an operand
+// are laid out [chunk][row] over the rows of this call.
+hipError_t stats_pass(const double* a, int64_t lda, int m, bool kca, uint64_t* max_a, double* ss_a, int* sig_a,
+                      const double* b, int64_t ldb, int n, bool kcb, uint64_t* max_b, double* ss_b, int* sig_b, int k,
+                      int nchunks, int alpha, bool crt, hipStream_t s) {
+    if (m + n <= 0) return hipSuccess;
+    auto op = [&](const double* x, int64_t ld, int rows, bool kc, uint64_t* mx, double* ss) {
+        StatOp o;
+        o.x = x; o.ld = ld; o.rows = rows; o.kcontig = kc ? 1 : 0; o.maxbits = mx; o.part = ss;
+        o.blocks = kc ? (rows + 63) / 64 : (rows + 255) / 256;
+        return o;
+    };
+    const StatOp oa = op(a, lda, m, kca, max_a, ss_a), ob = op(b, ldb, n, kcb, max_b, ss_b);
+    dim3 grid(oa.blocks + ob.blocks, nchunks);
+    row_stats_kernel<<<grid, 256, 0, s>>>(oa, ob, k);
+    const ExpOp ea{m, max_a, ss_a, sig_a}, eb{n, max_b, ss_b, sig_b};
+    row_exponent_kernel<<<(m + n + 255) / 256, 256, 0, s>>>(ea, eb, k, nchunks, alpha, crt ? 1 : 0);
+    return hipGetLastError();
+}
+
+// Planes of op(A) and op(B) in one launch.
+hipError_t planes2(const double* a, int64_t lda, int m, bool kca, const int* sig_a, int8_t* pa, int64_t plane_a,
+                   const double* b, int64_t ldb, int n, bool kcb, const int* sig_b, int8_t* pb, int64_t plane_b,
+                   int k, int kp, int nplanes, bool crt, const CrtConsts* cc, hipStream_t s, bool bf16) {
+    PlaneArgs pa_;
+    memset(&pa_, 0, sizeof(pa_));
+    auto op = [&](const double* x, int64_t ld, int rows, bool kc, const int* sig, int8_t* out, int64_t stride) {
+        PlaneOp o;
+        o.x = x; o.ld = ld; o.rows = rows; o.kcontig = kc ? 1 : 0; o.sigma = sig; o.out = out;
+        o.plane_stride = stride;
+        o.blocks = (rows + ZG_PLANE_ROWS - 1) / ZG_PLANE_ROWS;
+        return o;
+    };
+    pa_.op[0] = op(a, lda, m, kca, sig_a, pa, plane_a);
+    pa_.op[1] = op(b, ldb, n, kcb, sig_b, pb, plane_b);
+    pa_.k = k; pa_.kp = kp; pa_.nplanes = nplanes; pa_.crt = crt ? 1 : 0; pa_.bf16 = bf16 ? 1 : 0;
     if (crt) {
         for (int l = 0; l < nplanes; ++l) {
-            const int m = cc->m[l];
-            pa.m[l] = m;
-            pa.inv_m[l] = cc->inv_m[l];
-            const uint64_t c32 = (1ull << 32) % (uint64_t)m;
-            pa.c32[l] = (double)c32;
-            pa.c64[l] = (double)((c32 * c32) % (uint64_t)m);
+            const int md = cc->m[l];
+            pa_.m[l] = md;
+            pa_.inv_m[l] = cc->inv_m[l];
+            const uint64_t c32 = (1ull << 32) % (uint64_t)md;
+            pa_.c32[l] = (double)c32;
+            pa_.c64[l] = (double)((c32 * c32) % (uint64_t)md);
         }
     }
     // |a| <= 2^(alpha+1) fits two parts below 2^63
     const bool three = crt && cc->alpha + 1 >= 63;
-    dim3 grid((rows + ZG_PLANE_ROWS - 1) / ZG_PLANE_ROWS, kp / 128);
-    if (kcontig) {
-        if (three) plane_kernel<true, 3><<<grid, 256, 0, s>>>(pa);
-        else plane_kernel<true, 2><<<grid, 256, 0, s>>>(pa);
-    } else {
-        if (three) plane_kernel<false, 3><<<grid, 256, 0, s>>>(pa);
-        else plane_kernel<false, 2><<<grid, 256, 0, s>>>(pa);
-    }
+    if (pa_.op[0].blocks + pa_.op[1].blocks == 0) return hipSuccess;
+    dim3 grid(pa_.op[0].blocks + pa_.op[1].blocks, kp / kPlaneTK);
+    if (three) plane_kernel<3><<<grid, 256, 0, s>>>(pa_);
+    else plane_kernel<2><<<grid, 256, 0, s>>>(pa_);
     return hipGetLastError();
 }
 }  // namespace
@@ -1295,26 +1381,35 @@ int ozaki_dgemm(bool ta, bool tb, int m, int n, int k, double alpha, const doubl
     int8_t* pa = ws.take<int8_t>(plane_a * planes_n);
     int8_t* pb = ws.take<int8_t>(plane_b * planes_n);
 
-    hipError_t e = hipSuccess;
-    // The two operands' statistics and planes are independent: B's run on the
-    // auxiliary stream beside A's (both memory-bound, neither fills HBM alone)
-    // and are joined back before the products.
+    // op(A) rows are contiguous in k when transposed; op(B) "rows" (its
+    // columns) when not.  Each stage is one launch over both operands.
+    //
+    // Large CRT products run in ZG_CRT_PARTS parts of N on two streams: s
+    // prepares op(A) and op(B)'s first part and starts that part's INT8
+    // products (MFMA-bound) while the side stream prepares the rest of op(B)
+    // (memory-bound); after that, the side stream reconstructs each part while
+    // s runs the next part's products.  Every kernel computes what it
+    // computes unsplit, so the results are the same bits.
+    const int parts = (crt && n >= kCrtSplitMinN) ? kCrtParts : 1;
+    const int nfirst = (parts > 1) ? (n / parts + 255) / 256 * 256 : n;
+    const double* b_rest = tb ? b + nfirst : b + (int64_t)nfirst * ldb;  // op(B) columns nfirst..n
+    hipError_t e = stats_zero(max_a, m, max_b, n, s);
+    if (e == hipSuccess)
+        e = stats_pass(a, lda, m, ta, max_a, ss_a, sig_a, b, ldb, nfirst, !tb, max_b, ss_b, sig_b, k, nchunks,
+                       crt ? cc.alpha : 0, crt, s);
+    if (e == hipSuccess)
+        e = planes2(a, lda, m, ta, sig_a, pa, plane_a, b, ldb, nfirst, !tb, sig_b, pb, plane_b, k, kp, planes_n, crt,
+                    &cc, s, bf16);
     hipStream_t side = nullptr;
-    const bool forked = aux_fork(s, &side);
-    hipStream_t sb = forked ? side : s;
-    // op(A) rows are contiguous in k when transposed; op(B) "rows" (its columns) when not
-    e = operand_stats(a, lda, m, k, ta, max_a, ss_a, s);
-    if (e == hipSuccess) e = operand_stats(b, ldb, n, k, !tb, max_b, ss_b, sb);
-    if (e == hipSuccess) {
-        row_exponent_kernel<<<(m + 255) / 256, 256, 0, s>>>(m, k, max_a, ss_a, nchunks, cc.alpha, crt, sig_a);
-        row_exponent_kernel<<<(n + 255) / 256, 256, 0, sb>>>(n, k, max_b, ss_b, nchunks, cc.alpha, crt, sig_b);
-        e = hipGetLastError();
-    }
-    if (e == hipSuccess) e = planes(a, lda, m, k, kp, ta, sig_a, pa, plane_a, planes_n, crt, &cc, s, bf16);
-    if (e == hipSuccess) e = planes(b, ldb, n, k, kp, !tb, sig_b, pb, plane_b, planes_n, crt, &cc, sb, bf16);
-    if (forked) {
-        const hipError_t ej = aux_join(s, side);  // always joined: the workspace is released after s drains
-        if (e == hipSuccess) e = ej;
+    const bool split = (e == hipSuccess) && nfirst < n && aux_fork(s, &side);
+    const hipStream_t sr = split ? side : s;  // where the rest of op(B) is prepared
+    if (e == hipSuccess && nfirst < n) {
+        double* ss_rest = ss_b + (size_t)nfirst * nchunks;
+        e = stats_pass(nullptr, 0, 0, false, nullptr, nullptr, nullptr, b_rest, ldb, n - nfirst, !tb, max_b + nfirst,
+                       ss_rest, sig_b + nfirst, k, nchunks, crt ? cc.alpha : 0, crt, sr);
+        if (e == hipSuccess)
+            e = planes2(nullptr, 0, 0, false, nullptr, nullptr, 0, b_rest, ldb, n - nfirst, !tb, sig_b + nfirst,
+                        pb + (int64_t)nfirst * kp * eb, plane_b, k, kp, planes_n, crt, &cc, sr, bf16);
     }
 
     I8Args ia;
@@ -1332,22 +1427,55 @@ int ozaki_dgemm(bool ta, bool tb, int m, int n, int k, double alpha, const doubl
             ia.outs[l].y = cc.y[l];
             ia.outs[l].inv_m = cc.inv_m[l];
         }
-        e = i8_gemm_launch(ia, planes_n, s, tile >= 0 ? tile : i8_pick_tile(m, n, kp));
-        if (e == hipSuccess) {
+        // products and reconstruction of output columns [c0, c0 + nc)
+        auto products = [&](int c0, int nc) {
+            I8Args h = ia;
+            h.n = nc;
+            h.b = pb + (int64_t)c0 * kp * eb;
+            h.u = u + (int64_t)c0 * ldu;
+            return i8_gemm_launch(h, planes_n, s, tile >= 0 ? tile : i8_pick_tile(m, nc, kp));
+        };
+        auto reconstruct = [&](int c0, int nc, hipStream_t st) {
             CrtArgs ca;
-            ca.m = m; ca.n = n; ca.u = u; ca.u_plane = ia.u_plane; ca.ldu = ldu;
-            ca.sig_a = sig_a; ca.sig_b = sig_b; ca.alpha = alpha; ca.beta = beta; ca.c = c; ca.ldc = ldc;
+            ca.m = m; ca.n = nc; ca.u = u + (int64_t)c0 * ldu; ca.u_plane = ia.u_plane; ca.ldu = ldu;
+            ca.sig_a = sig_a; ca.sig_b = sig_b + c0; ca.alpha = alpha; ca.beta = beta; ca.c = c + (int64_t)c0 * ldc;
+            ca.ldc = ldc;
             ca.cc = cc;
-            dim3 grid((m + kCrtThreads * kCrtRows - 1) / (kCrtThreads * kCrtRows), n);
+            dim3 grid((m + kCrtThreads * kCrtRows - 1) / (kCrtThreads * kCrtRows), nc);
             switch (cc.ndig) {
-                case 2: crt_reconstruct_kernel<2><<<grid, kCrtThreads, 0, s>>>(ca); break;
-                case 3: crt_reconstruct_kernel<3><<<grid, kCrtThreads, 0, s>>>(ca); break;
-                case 4: crt_reconstruct_kernel<4><<<grid, kCrtThreads, 0, s>>>(ca); break;
-                default: crt_reconstruct_kernel<5><<<grid, kCrtThreads, 0, s>>>(ca); break;
+                case 2: crt_reconstruct_kernel<2><<<grid, kCrtThreads, 0, st>>>(ca); break;
+                case 3: crt_reconstruct_kernel<3><<<grid, kCrtThreads, 0, st>>>(ca); break;
+                case 4: crt_reconstruct_kernel<4><<<grid, kCrtThreads, 0, st>>>(ca); break;
+                default: crt_reconstruct_kernel<5><<<grid, kCrtThreads, 0, st>>>(ca); break;
             }
-            e = hipGetLastError();
+            return hipGetLastError();
+        };
+        if (!split) {
+            e = products(0, n);
+            if (e == hipSuccess) e = reconstruct(0, n, s);
+        } else {
+            // part p covers columns [p * nfirst, min(n, (p + 1) * nfirst))
+            for (int c0 = 0; c0 < n && e == hipSuccess; c0 += nfirst) {
+                const int nc = std::min(nfirst, n - c0);
+                e = products(c0, nc);
+                if (e != hipSuccess) break;
+                if (c0 == 0) e = aux_join(s, side);  // the later parts' planes are ready
+                if (e != hipSuccess) break;
+                if (c0 + nc < n) {
+                    hipStream_t sd = nullptr;  // the side stream, after this part's products
+                    if (!aux_fork(s, &sd)) e = hipErrorUnknown;
+                    if (e == hipSuccess) e = reconstruct(c0, nc, sd);
+                } else {
+                    e = reconstruct(c0, nc, s);
+                }
+            }
         }
-    } else if (e == hipSuccess) {
+    }
+    if (split) {
+        const hipError_t ej = aux_join(s, side);  // always joined: the workspace is released after s drains
+        if (e == hipSuccess) e = ej;
+    }
+    if (!crt && e == hipSuccess) {
         double* w = ws.take<double>((size_t)m * n);
         ia.mode = 1;
         ia.w = w; ia.ldw = m;
@@ -1375,7 +1503,7 @@ int ozaki_dgemm(bool ta, bool tb, int m, int n, int k, double alpha, const doubl
     int st = status_from_hip(e);
     const hipError_t es = hipStreamSynchronize(s);  // workspace reuse is host-ordered
     if (st == ZYNUM_GPU_OK) st = status_from_hip(es);
-    if (forked && st != ZYNUM_GPU_OK) (void)hipStreamSynchronize(side);
+    if (split && st != ZYNUM_GPU_OK) (void)hipStreamSynchronize(side);
     workspace_release();
     return st;
 }
