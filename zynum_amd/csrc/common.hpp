@@ -42,12 +42,35 @@ template <> struct Mfma<float> {
 __device__ __forceinline__ float fma_t(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 __device__ __forceinline__ double fma_t(double a, double b, double c) { return __builtin_fma(a, b, c); }
 
+// Pointers into global memory by address space.  A pointer read from a
+// descriptor in memory is generic, and accesses through it are flat
+// instructions, which count against lgkmcnt as well as vmcnt: an LDS wait then
+// also waits for them.  Accesses through a gptr are global instructions.
+template <typename T>
+using gptr = __attribute__((address_space(1))) T*;
+template <typename T>
+__device__ __forceinline__ gptr<T> to_global(T* p) {
+    return (gptr<T>)p;
+}
+
 // Zynum's alpha/beta write-back (epilogue.zig:10-39):
 //   alpha == 1 && beta == 0 -> acc
 //   otherwise fma(alpha, acc, s), s = 0 (beta == 0), old (beta == 1), RN(beta * old)
 // C is not read when beta == 0, so NaNs in C never propagate.
 template <typename T>
 __device__ __forceinline__ T epilogue(T alpha, T beta, T acc, const T* cptr) {
+    if (alpha == T(1) && beta == T(0)) return acc;
+    T s = T(0);
+    if (beta != T(0)) {
+        T old = *cptr;
+        s = (beta == T(1)) ? old : beta * old;
+    }
+    return fma_t(alpha, acc, s);
+}
+
+// The same write-back through a global pointer.
+template <typename T>
+__device__ __forceinline__ T epilogue_g(T alpha, T beta, T acc, gptr<const T> cptr) {
     if (alpha == T(1) && beta == T(0)) return acc;
     T s = T(0);
     if (beta != T(0)) {

@@ -33,8 +33,9 @@ struct Stage {
 
     T v[E];
 
-    __device__ __forceinline__ void load(const T* __restrict__ x, int64_t ldx, int r0, int k0, int rlim,
+    __device__ __forceinline__ void load(const T* __restrict__ xp, int64_t ldx, int r0, int k0, int rlim,
                                          int klim, bool full, int tid) {
+        const gptr<const T> x = to_global(xp);  // global loads also for a pointer read from memory
 #pragma unroll
         for (int i = 0; i < NV; ++i) {
             const int idx = tid + NT * i;
@@ -47,7 +48,7 @@ struct Stage {
                 if constexpr (VEC == 1) {
                     v[i] = x[off];
                 } else {
-                    vec_t<T, VEC> t = *reinterpret_cast<const vec_t<T, VEC>*>(x + off);
+                    vec_t<T, VEC> t = *(gptr<const vec_t<T, VEC>>)(x + off);
 #pragma unroll
                     for (int e = 0; e < VEC; ++e) v[i * VEC + e] = t[e];
                 }

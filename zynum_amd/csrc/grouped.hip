@@ -22,6 +22,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <type_traits>
 #include <vector>
 
 #include "common.hpp"
@@ -62,12 +63,14 @@ constexpr int kShapeCols[4] = {64, 32, 64, 128};
 
 // One persistent kernel per tile path (shape and transposes), so each runs
 // with only its own registers and LDS.  A workgroup walks its share of the
-// path's LPT list round-robin.  Tiles are software-pipelined across tile
-// boundaries: the last K step of a tile already loads the first K slice of the
-// workgroup's next tile, so that tile's operand latency overlaps this one's
-// last MFMAs and epilogue.  The per-element arithmetic (16x16x4 MFMA steps in
-// increasing k, then the alpha/beta epilogue) is the plain GEMM tile's, so
-// results are bitwise those of single GEMM calls.
+// path's LPT list round-robin.  Its K slices form one stream across tile
+// boundaries, loaded two slices ahead of the one being multiplied (two
+// register sets, alternating with the two LDS buffers): a slice's loads have
+// two slices of MFMA work to land, and a tile's first slices are already in
+// flight while the previous tile finishes and stores its results.  The
+// per-element arithmetic (16x16x4 MFMA steps in increasing k, then the
+// alpha/beta epilogue) is the plain GEMM tile's, so results are bitwise those
+// of single GEMM calls.
 template <typename T, class CF, bool TA, bool TB, int VEC>
 __global__ __launch_bounds__(CF::NT) __attribute__((amdgpu_waves_per_eu(CF::OCC))) void grouped_path_kernel(const TileItem<T>* __restrict__ items, int nitems) {
     constexpr int BM = CF::BM, BN = CF::BN, BK = CF::BK, NT = CF::NT;
@@ -84,19 +87,37 @@ __global__ __launch_bounds__(CF::NT) __attribute__((amdgpu_waves_per_eu(CF::OCC)
     const int wm0 = (wave % CF::WM) * WTM, wn0 = (wave / CF::WM) * WTN;
     const int lr = lane & 15, lk = lane >> 4;
     const int ar = M::a_row(lr);
+    const int G = gridDim.x;
 
-    int it = blockIdx.x;
-    if (it >= nitems) return;
+    int cit = blockIdx.x;  // tile being multiplied
+    if (cit >= nitems) return;
     // vector width per operand: at most the elements each thread stages
     constexpr int VA = (VEC < BM * BK / NT) ? VEC : BM * BK / NT;
     constexpr int VB = (VEC < BN * BK / NT) ? VEC : BN * BK / NT;
-    Stage<T, BM, BK, TA, VA, NT, PAD> sa;
-    Stage<T, BN, BK, !TB, VB, NT, PAD> sb;
-    auto load_slice = [&](const TileItem<T>& w, int k0) {
-        const GemmArgs<T>& p = w.p;
-        const bool full = (w.m0 + BM <= p.m) && (w.n0 + BN <= p.n) && (k0 + BK <= p.k);
-        sa.load(p.a, p.lda, w.m0, k0, p.m, p.k, full, tid);
-        sb.load(p.b, p.ldb, w.n0, k0, p.n, p.k, full, tid);
+    Stage<T, BM, BK, TA, VA, NT, PAD> sa[2];
+    Stage<T, BN, BK, !TB, VB, NT, PAD> sb[2];
+    using Set0 = std::integral_constant<int, 0>;
+    using Set1 = std::integral_constant<int, 1>;
+
+    // load cursor: the next slice to load is K offset lk0 of tile lit
+    // (descriptor lw); lmore is false once the stream is exhausted.
+    // Descriptors are uniform (scalar loads).
+    int lit = cit, lk0 = 0;
+    TileItem<T> lw = items[lit];
+    bool lmore = true;
+    auto load_next = [&](auto set_c) {
+        constexpr int S = decltype(set_c)::value;
+        const GemmArgs<T>& p = lw.p;
+        const bool full = (lw.m0 + BM <= p.m) && (lw.n0 + BN <= p.n) && (lk0 + BK <= p.k);
+        sa[S].load(p.a, p.lda, lw.m0, lk0, p.m, p.k, full, tid);
+        sb[S].load(p.b, p.ldb, lw.n0, lk0, p.n, p.k, full, tid);
+        lk0 += BK;
+        if (lk0 >= p.k) {
+            lit += G;
+            lk0 = 0;
+            lmore = lit < nitems;
+            if (lmore) lw = items[lit];
+        }
     };
     auto barrier = [&]() {  // LDS reads and writes of this step done; global loads may stay in flight
         asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
@@ -104,68 +125,88 @@ __global__ __launch_bounds__(CF::NT) __attribute__((amdgpu_waves_per_eu(CF::OCC)
         asm volatile("" ::: "memory");  // no LDS access moves above the barrier
     };
 
-    // Descriptors are uniform (scalar loads); the next one is read only where
-    // it is used, so it does not hold registers through the tile.
-    TileItem<T> cur = items[it];
-    load_slice(cur, 0);
-    for (;;) {
-        const int nx = it + gridDim.x;
-        const bool has_next = nx < nitems;
-        const GemmArgs<T>& p = cur.p;
-        const int m0 = cur.m0, n0 = cur.n0;
-        const int nk = (p.k + BK - 1) / BK;
-        sa.store(lds_a, tid);
-        sb.store(lds_b, tid);
-        barrier();
-        Acc acc[FM][FN];
+    // compute cursor: slice ct of the cnk slices of tile cit (descriptor cw)
+    TileItem<T> cw = lw;
+    int ct = 0, cnk = (cw.p.k + BK - 1) / BK;
+    Acc acc[FM][FN];
+    auto zero = [&]() {
 #pragma unroll
         for (int i = 0; i < FM; ++i)
 #pragma unroll
             for (int j = 0; j < FN; ++j) acc[i][j] = M::zero();
-        for (int t = 0; t < nk; ++t) {
-            const int cb = t & 1;
-            const bool more = (t + 1) < nk;
-            if (more) load_slice(cur, (t + 1) * BK);
-            else if (has_next) load_slice(items[nx], 0);
-            const T* __restrict__ la = lds_a + cb * (BK * (BM + PAD));
-            const T* __restrict__ lb = lds_b + cb * (BK * (BN + PAD));
-#pragma unroll
-            for (int kk = 0; kk < BK; kk += 4) {
-                T af[FM], bf[FN];
-#pragma unroll
-                for (int i = 0; i < FM; ++i) af[i] = la[(kk + lk) * (BM + PAD) + wm0 + i * 16 + ar];
-#pragma unroll
-                for (int j = 0; j < FN; ++j) bf[j] = lb[(kk + lk) * (BN + PAD) + wn0 + j * 16 + lr];
-#pragma unroll
-                for (int i = 0; i < FM; ++i)
-#pragma unroll
-                    for (int j = 0; j < FN; ++j) acc[i][j] = M::mma(af[i], bf[j], acc[i][j]);
-            }
-            if (more) {
-                sa.store(lds_a + (cb ^ 1) * (BK * (BM + PAD)), tid);
-                sb.store(lds_b + (cb ^ 1) * (BK * (BN + PAD)), tid);
-            }
-            barrier();
-        }
-        // epilogue: lane holds rows 4*(lane/16)+r, column lane%16 of each block
+    };
+    auto epilogue_store = [&]() {
+        // lane holds rows 4*(lane/16)+r, column lane%16 of each block
+        const GemmArgs<T>& p = cw.p;
         const T alpha = p.alpha, beta = p.beta;
 #pragma unroll
         for (int i = 0; i < FM; ++i) {
 #pragma unroll
             for (int j = 0; j < FN; ++j) {
-                const int col = n0 + wn0 + j * 16 + lr;
-                const int row = m0 + wm0 + i * 16 + 4 * lk;
+                const int col = cw.n0 + wn0 + j * 16 + lr;
+                const int row = cw.m0 + wm0 + i * 16 + 4 * lk;
                 if (col < p.n) {
-                    T* cp = p.c + (int64_t)col * p.ldc + row;
+                    const gptr<T> cp = to_global(p.c) + (int64_t)col * p.ldc + row;
 #pragma unroll
                     for (int r = 0; r < 4; ++r)
-                        if (row + r < p.m) cp[r] = epilogue<T>(alpha, beta, acc[i][j][r], cp + r);
+                        if (row + r < p.m) cp[r] = epilogue_g<T>(alpha, beta, acc[i][j][r], cp + r);
                 }
             }
         }
-        if (!has_next) return;
-        cur = items[nx];
-        it = nx;
+    };
+    auto compute = [&](int cb) {
+        const T* __restrict__ la = lds_a + cb * (BK * (BM + PAD));
+        const T* __restrict__ lb = lds_b + cb * (BK * (BN + PAD));
+#pragma unroll
+        for (int kk = 0; kk < BK; kk += 4) {
+            T af[FM], bf[FN];
+#pragma unroll
+            for (int i = 0; i < FM; ++i) af[i] = la[(kk + lk) * (BM + PAD) + wm0 + i * 16 + ar];
+#pragma unroll
+            for (int j = 0; j < FN; ++j) bf[j] = lb[(kk + lk) * (BN + PAD) + wn0 + j * 16 + lr];
+#pragma unroll
+            for (int i = 0; i < FM; ++i)
+#pragma unroll
+                for (int j = 0; j < FN; ++j) acc[i][j] = M::mma(af[i], bf[j], acc[i][j]);
+        }
+    };
+
+    // prologue: stream positions 0 and 1 into sets 0 and 1, position 0 into LDS buffer 0
+    load_next(Set0{});
+    bool next_loaded = lmore;  // position 1 exists
+    if (next_loaded) load_next(Set1{});
+    sa[0].store(lds_a, tid);
+    sb[0].store(lds_b, tid);
+    barrier();
+    zero();
+    // One stream position with its operands in register set / LDS buffer P;
+    // returns false when the stream is done.
+    auto step = [&](auto par_c) -> bool {
+        constexpr int P = decltype(par_c)::value;
+        using Own = std::integral_constant<int, P>;
+        const bool loaded2 = lmore;  // position q + 2 exists
+        if (loaded2) load_next(Own{});  // set P's operands are in LDS buffer P already
+        compute(P);
+        if (next_loaded) {
+            sa[1 - P].store(lds_a + (1 - P) * (BK * (BM + PAD)), tid);
+            sb[1 - P].store(lds_b + (1 - P) * (BK * (BN + PAD)), tid);
+        }
+        barrier();
+        next_loaded = loaded2;
+        if (++ct == cnk) {
+            epilogue_store();
+            cit += G;
+            if (cit >= nitems) return false;
+            cw = items[cit];
+            ct = 0;
+            cnk = (cw.p.k + BK - 1) / BK;
+            zero();
+        }
+        return true;
+    };
+    for (;;) {
+        if (!step(Set0{})) break;
+        if (!step(Set1{})) break;
     }
 }
 
@@ -177,7 +218,7 @@ __global__ __launch_bounds__(256) void grouped_scale_kernel(const TileItem<T>* _
         for (int e = threadIdx.x; e < kScaleTile * kScaleTile; e += 256) {
             const int i = w.m0 + e % kScaleTile, j = w.n0 + e / kScaleTile;
             if (i < w.p.m && j < w.p.n) {
-                T* cp = w.p.c + (int64_t)j * w.p.ldc + i;
+                const gptr<T> cp = to_global(w.p.c) + (int64_t)j * w.p.ldc + i;
                 *cp = (w.p.beta == T(0)) ? T(0) : w.p.beta * *cp;
             }
         }
