@@ -119,6 +119,30 @@ def test_dtrsm_residual_random(torch_cuda, oracle, side, uplo, trans, diag):
     assert np.max(np.abs(X - Xr)) <= 64 * na * 2.0**-53 * cond_scale
 
 
+@pytest.mark.parametrize("side,uplo,trans", [(s, u, t) for s in "LR" for u in "LU" for t in "NT"])
+def test_dtrsm_lookahead_integer_exact_recovery(torch_cuda, side, uplo, trans):
+    """A triangle of 4160 (second halves of 2048 and more: the plain form
+    overlaps the far part of each such update with the near part's solve on a
+    second stream): integer systems recover X bitwise in every side, uplo and
+    transpose form, so no update runs before the unknowns it reads are final."""
+    rng = np.random.default_rng(sum(ord(ch) * 31**i for i, ch in enumerate("la" + side + uplo + trans)))
+    na, other = 4160, 72
+    m, n = (na, other) if side == "L" else (other, na)
+    A = rng.integers(-1, 2, (na, na)).astype(np.float64)
+    T = (np.tril(A, -1) if uplo == "L" else np.triu(A, 1)) + np.eye(na)
+    opA = T.T if trans == "T" else T
+    X = rng.integers(-3, 4, (m, n)).astype(np.float64)
+    Bm = opA @ X if side == "L" else X @ opA
+    assert np.max(np.abs(Bm)) < 2**40
+    lda, ldb = na + 1, m + 3
+    a = triangle_storage(T, uplo, lda)
+    bb = np.full((n, ldb), -777.0)
+    bb[:, :m] = Bm.T
+    got = trsm(torch_cuda, side, uplo, trans, "N", m, n, 1.0, a, lda, bb.ravel(), ldb).reshape(n, ldb)
+    assert np.array_equal(got[:, :m].T, X)
+    assert np.all(got[:, m:] == -777.0)
+
+
 LEAF_CASES = ["plain", "wide_range", "tiny_column", "huge_diagonal", "zeros", "padded"]
 
 

@@ -301,6 +301,10 @@ struct TrsmCtx {
     int64_t ldb;
     hipStream_t s;
     int mode, mode_param;
+    // lookahead (plain updates only): the second stream and the call's own
+    // fork / join events; null when not used
+    hipStream_t side = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
 };
 
 // op(A)[r0.., c0..] as a GEMM operand (pointer, transpose flag)
@@ -361,46 +365,79 @@ int leaf(const TrsmCtx<T>& c, int i0, int nb) {
 }
 
 // solves the unknowns [i0, i0 + len) (rows of X for side L, columns for side R)
+// A range of unknowns (rows of X for side L, columns for side R).
+struct Range {
+    int lo, len;
+};
+
+// The two halves of the unknowns [i0, i0 + len) in solve order: f is solved
+// first.  Forward when the equations are lower triangular (side L with op(A)
+// lower, side R with op(A) upper).
 template <typename T>
-int solve(const TrsmCtx<T>& c, int i0, int len) {
-    if (len <= kLeaf) return leaf(c, i0, len);
+void halves(const TrsmCtx<T>& c, int i0, int len, Range& f, Range& g) {
     int h = (len / 2 + kLeaf - 1) / kLeaf * kLeaf;
     if (h >= len) h = len - kLeaf;
-    const int i1 = i0 + h, len2 = len - h;
-    // forward when the equations are lower triangular: side L with op(A) lower,
-    // side R with op(A) upper
     const bool forward = c.left ? c.lower_eff : !c.lower_eff;
-    int st;
-    if (c.left) {
-        if (forward) {
-            if ((st = solve(c, i0, h))) return st;
-            // B[i1:, :] -= op(A)[i1:, i0:i1] X[i0:i1, :]
-            st = gemm_update(c, c.trans, false, len2, c.n, h, sub_opa(c, i1, i0), c.lda, c.b + i0, c.ldb, c.b + i1,
-                             c.ldb);
-            if (st) return st;
-            return solve(c, i1, len2);
-        }
-        if ((st = solve(c, i1, len2))) return st;
-        // B[i0:i1, :] -= op(A)[i0:i1, i1:] X[i1:, :]
-        st = gemm_update(c, c.trans, false, h, c.n, len2, sub_opa(c, i0, i1), c.lda, c.b + i1, c.ldb, c.b + i0,
-                         c.ldb);
-        if (st) return st;
-        return solve(c, i0, h);
-    }
     if (forward) {
-        if ((st = solve(c, i0, h))) return st;
-        // B[:, i1:] -= X[:, i0:i1] op(A)[i0:i1, i1:]
-        st = gemm_update(c, false, c.trans, c.m, len2, h, c.b + (int64_t)i0 * c.ldb, c.ldb, sub_opa(c, i0, i1),
-                         c.lda, c.b + (int64_t)i1 * c.ldb, c.ldb);
-        if (st) return st;
-        return solve(c, i1, len2);
+        f = {i0, h};
+        g = {i0 + h, len - h};
+    } else {
+        f = {i0 + h, len - h};
+        g = {i0, h};
     }
-    if ((st = solve(c, i1, len2))) return st;
-    // B[:, i0:i1] -= X[:, i1:] op(A)[i1:, i0:i1]
-    st = gemm_update(c, false, c.trans, c.m, h, len2, c.b + (int64_t)i1 * c.ldb, c.ldb, sub_opa(c, i1, i0), c.lda,
-                     c.b + (int64_t)i0 * c.ldb, c.ldb);
-    if (st) return st;
-    return solve(c, i0, h);
+}
+
+// Unknowns r -= coupling to the solved unknowns s, on stream st:
+// side L, B[r, :] -= op(A)[r, s] X[s, :]; side R, B[:, r] -= X[:, s] op(A)[s, r].
+template <typename T>
+int update_range(const TrsmCtx<T>& c, Range r, Range s, hipStream_t st) {
+    TrsmCtx<T> cs = c;
+    cs.s = st;
+    if (c.left)
+        return gemm_update(cs, c.trans, false, r.len, c.n, s.len, sub_opa(c, r.lo, s.lo), c.lda, c.b + s.lo, c.ldb,
+                           c.b + r.lo, c.ldb);
+    return gemm_update(cs, false, c.trans, c.m, r.len, s.len, c.b + (int64_t)s.lo * c.ldb, c.ldb,
+                       sub_opa(c, s.lo, r.lo), c.lda, c.b + (int64_t)r.lo * c.ldb, c.ldb);
+}
+
+// Smallest second half that is solved with lookahead.
+constexpr int kTrsmLookaheadMin = 2048;
+
+// Solves the unknowns [i0, i0 + len): the first half, the update of the
+// second half, the second half (structured_blocked.zig:315-374 blocks the
+// same way).  With lookahead (a second stream, plain updates, a second half
+// of at least kTrsmLookaheadMin), the second half's update is split at the
+// second half's own first half g1: g1's part runs on the main stream and is
+// solved there while the rest of the update (g2's part, the larger GEMM)
+// runs on the second stream; the main stream waits for it only before g2's
+// next update.  The leaf solves and thin updates inside g1 are latency-bound
+// and leave most of the chip to that GEMM.  Every update writes what it
+// writes without lookahead.  A lookahead is not nested inside g1's solve
+// (allow_la), so one update at a time is on the second stream.
+template <typename T>
+int solve(const TrsmCtx<T>& c, int i0, int len, bool allow_la = true) {
+    if (len <= kLeaf) return leaf(c, i0, len);
+    Range f, g;
+    halves(c, i0, len, f, g);
+    int st;
+    if ((st = solve(c, f.lo, f.len, allow_la))) return st;
+    if (!(allow_la && c.side != nullptr && g.len >= kTrsmLookaheadMin)) {
+        if ((st = update_range(c, g, f, c.s))) return st;
+        return solve(c, g.lo, g.len, allow_la);
+    }
+    Range g1, g2;
+    halves(c, g.lo, g.len, g1, g2);
+    // g2's part of the update on the second stream, after everything queued so far
+    if ((st = status_from_hip(hipEventRecord(c.ev_fork, c.s)))) return st;
+    if ((st = status_from_hip(hipStreamWaitEvent(c.side, c.ev_fork, 0)))) return st;
+    if ((st = update_range(c, g2, f, c.side))) return st;
+    if ((st = status_from_hip(hipEventRecord(c.ev_join, c.side)))) return st;
+    // g1's part and its solve on the main stream, beside it
+    if ((st = update_range(c, g1, f, c.s))) return st;
+    if ((st = solve(c, g1.lo, g1.len, false))) return st;
+    if ((st = status_from_hip(hipStreamWaitEvent(c.s, c.ev_join, 0)))) return st;
+    if ((st = update_range(c, g2, g1, c.s))) return st;
+    return solve(c, g2.lo, g2.len, allow_la);
 }
 }  // namespace
 
@@ -422,7 +459,21 @@ int trsm_device(char side, char uplo, bool trans, char diag, int m, int n, T alp
     c.unit = (diag == 'U');
     c.m = m; c.n = n; c.a = a; c.lda = lda; c.b = b; c.ldb = ldb; c.s = s;
     c.mode = mode; c.mode_param = mode_param;
-    return solve(c, 0, c.left ? m : n);
+    const int len = c.left ? m : n;
+    // lookahead needs a second half of at least kTrsmLookaheadMin below the top
+    if (mode == ZYNUM_GPU_MODE_PLAIN && len >= 2 * kTrsmLookaheadMin) {
+        hipStream_t side = aux_stream();
+        if (side != nullptr && hipEventCreateWithFlags(&c.ev_fork, hipEventDisableTiming) == hipSuccess &&
+            hipEventCreateWithFlags(&c.ev_join, hipEventDisableTiming) == hipSuccess) {
+            c.side = side;
+        }
+        (void)hipGetLastError();
+    }
+    const int st = solve(c, 0, len);
+    // (destroying an event the device has yet to reach is deferred by the runtime)
+    if (c.ev_fork) (void)hipEventDestroy(c.ev_fork);
+    if (c.ev_join) (void)hipEventDestroy(c.ev_join);
+    return st;
 }
 
 // ---------------------------------------------------------------------------

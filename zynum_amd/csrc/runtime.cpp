@@ -446,6 +446,7 @@ struct AuxStream {
 };
 constexpr int kAuxDevices = 64;
 AuxStream g_aux[kAuxDevices];
+std::mutex g_aux_mu;  // creation only: callers order their use of the stream with events
 
 AuxStream* aux_for_current_device() {
     int dev = 0;
@@ -454,6 +455,7 @@ AuxStream* aux_for_current_device() {
         return nullptr;
     }
     AuxStream& a = g_aux[dev];
+    std::lock_guard<std::mutex> lock(g_aux_mu);
     if (a.stream == nullptr) {
         hipStream_t st = nullptr;
         hipEvent_t f = nullptr, j = nullptr;
@@ -483,6 +485,11 @@ bool aux_fork(hipStream_t s, hipStream_t* side) {
     }
     *side = a->stream;
     return true;
+}
+
+hipStream_t aux_stream() {
+    AuxStream* a = aux_for_current_device();
+    return a ? a->stream : nullptr;
 }
 
 hipError_t aux_join(hipStream_t s, hipStream_t side) {

@@ -80,6 +80,9 @@ void scratch_release(void* p, size_t bytes);
 // when the stream or events cannot be created.
 bool aux_fork(hipStream_t s, hipStream_t* side);
 hipError_t aux_join(hipStream_t s, hipStream_t side);
+// The same second stream, for a caller that orders it with events of its own
+// (null when it cannot be created).
+hipStream_t aux_stream();
 
 int status_from_hip(hipError_t e);
 
