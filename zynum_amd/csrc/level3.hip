@@ -70,8 +70,9 @@ __device__ __forceinline__ T op_a(const LeafArgs<T>& p, int r, int c) {
 // diagonal in every lane and eliminates it, one FMA per later unknown of the
 // lane, in the reference's per-column order x_k /= a_kk;
 // x_i = fma(a_ik, -x_k, x_i) (matrix_vector/triangular.zig:375-385).  The
-// coefficients of a step are laid out in LDS per lane position,
-// Cs[k][q][j] = a_{4j+q, k}, and read two steps ahead of their use; the
+// coefficients of a step are laid out in LDS per lane position and negated,
+// Cs[k][q][j] = -a_{4j+q, k} (so no negation of x_k sits between the
+// division and the FMAs), and read two steps ahead of their use; the
 // diagonals are read as broadcasts.  A lane does a quarter of the FMAs and
 // reads a quarter of the coefficients of the one-lane-per-vector form, and
 // the grid has four times the waves: with one wave per SIMD the solve is
@@ -113,16 +114,6 @@ __device__ __forceinline__ float quad_bcast<float>(float v, int src) {
     return __int_as_float(x);
 }
 
-// -v as an opaque register value: otherwise the negation is sunk into each of
-// the FMAs that use it, one code-generation pass at a time (minutes of
-// compile time for the same instructions)
-template <typename T>
-__device__ __forceinline__ T neg_opaque(T v) {
-    T n = -v;
-    asm volatile("" : "+v"(n));
-    return n;
-}
-
 // coefficients (and diagonal) of step K into ring slot K % (kLeafAhead + 1);
 // cq points at this lane's Cs[0][q][0]
 template <typename T, bool UNIT, int K>
@@ -138,18 +129,21 @@ __device__ __forceinline__ void leaf_fetch(T (&c)[kLeafAhead + 1][kLeafU], T (&d
 
 template <typename T, bool UNIT, int K>
 __device__ __forceinline__ void leaf_step(T (&x)[kLeafU], T (&c)[kLeafAhead + 1][kLeafU], T (&d)[kLeafAhead + 1],
-                                          const T* cq, const T* Dg, int q) {
+                                          const T* cq, const T* Dg, int q, T& upd) {
     leaf_fetch<T, UNIT, K + kLeafAhead>(c, d, cq, Dg);
     constexpr int slot = K % (kLeafAhead + 1), jo = K / kQuad, o = K % kQuad;
-    T xk = quad_bcast<T>(x[jo], o);  // x_k from its owner lane
+    // x_k from its owner lane: for o > 0 the previous step's update of slot
+    // jo, which is what lane o kept there (q > o - 1), taken before the selects
+    T xk = quad_bcast<T>(o == 0 ? x[jo] : upd, o);
     if constexpr (!UNIT) xk = xk / d[slot];
-    const T nxk = neg_opaque(xk);
+    // the coefficients are stored negated: fma(-a_ik, x_k, x_i) is
+    // fma(a_ik, -x_k, x_i) bit for bit
 #pragma unroll
-    for (int j = jo + 1; j < kLeafU; ++j) x[j] = fma_t(c[slot][j], nxk, x[j]);
+    for (int j = jo + 1; j < kLeafU; ++j) x[j] = fma_t(c[slot][j], xk, x[j]);
     // slot jo: unknown k in the owner, earlier (solved) unknowns below it,
     // later ones above
     // (unpredictable: selects, not branches around the FMA and the division)
-    const T upd = fma_t(c[slot][jo], nxk, x[jo]);
+    upd = fma_t(c[slot][jo], xk, x[jo]);
     const T keep = __builtin_unpredictable(q > o) ? upd : x[jo];
     x[jo] = __builtin_unpredictable(q == o) ? xk : keep;
     // one scheduling region per step: the reads stay ahead, register pressure
@@ -168,12 +162,74 @@ __device__ __forceinline__ void leaf_solve(T (&x)[kLeafU], const T* cq, const T*
         for (int j = s / kQuad; j < kLeafU; ++j) c[s][j] = cq[s * kLdsK + j];
         if constexpr (!UNIT) d[s] = Dg[s];
     }
-    (leaf_step<T, UNIT, K>(x, c, d, cq, Dg, q), ...);
+    T upd = T(0);
+    (leaf_step<T, UNIT, K>(x, c, d, cq, Dg, q, upd), ...);
+}
+
+// Coefficients of the 64-block whose first stored element is ablk: thread t
+// takes stored row t, av[c] = stored column c (the wave reads one contiguous
+// column per load), and dv = the diagonal of equation t.  Every load is
+// issued before the first is used.  A full block (nb = 64) needs no clamping;
+// otherwise rows and columns past nb repeat the last one.
+template <typename T, typename P>
+__device__ __forceinline__ void leaf_coeffs_load(T (&av)[kLeaf], T& dv, P ablk, int64_t lda, int nb, int t) {
+    const int tc = min(t, nb - 1);
+    if (nb == kLeaf) {
+        P col = ablk + t;
+#pragma unroll
+        for (int c = 0; c < kLeaf; ++c, col += lda) av[c] = *col;
+    } else {
+#pragma unroll
+        for (int c = 0; c < kLeaf; ++c) av[c] = ablk[(int64_t)min(c, nb - 1) * lda + tc];
+    }
+    dv = ablk[(int64_t)tc * lda + tc];
+}
+
+// The block's coefficients into LDS in solve order, negated,
+// Cs[k][q][j] = -M_{4j+q, k} (M the equation matrix of the block), and its
+// diagonal into Dg.  Side L
+// solves op(A) x = b (M = op(A) block), side R solves x op(A) = b, i.e.
+// op(A)^T x^T = b^T (M = op(A)^T block), so the stored element (row r,
+// column c) is M[r][c] for side L without transpose and for side R with it,
+// and M[c][r] otherwise (swap).  Every (equation, unknown) pair has its own
+// slot.  A full block needs no zeroing: every slot a step reads for a later
+// unknown holds its coefficient, and the other slots only feed updates that
+// the owner selection discards.  A partial block zeroes the slots of padding
+// and of the other triangle; padding equations get diagonal 1.
+template <typename T, bool FWD, bool UNIT>
+__device__ __forceinline__ void leaf_coeffs_store(T* Cs, T* Dg, const T (&av)[kLeaf], T dv, int nb, int t,
+                                                  bool swap) {
+    if constexpr (!UNIT) Dg[FWD ? t : kLeaf - 1 - t] = (t < nb) ? dv : T(1);
+    auto slot = [](int i, int k) {  // slot of M[i][k]
+        const int ui = FWD ? i : kLeaf - 1 - i, uk = FWD ? k : kLeaf - 1 - k;
+        return uk * kLdsK + (ui % kQuad) * kLdsQ + ui / kQuad;
+    };
+    auto keep = [&](int i, int k) {  // a coefficient some step reads
+        const int ui = FWD ? i : kLeaf - 1 - i, uk = FWD ? k : kLeaf - 1 - k;
+        return i < nb && k < nb && ui > uk;
+    };
+    if (nb == kLeaf) {
+        if (swap) {
+#pragma unroll
+            for (int c = 0; c < kLeaf; ++c) Cs[slot(c, t)] = -av[c];
+        } else {
+#pragma unroll
+            for (int c = 0; c < kLeaf; ++c) Cs[slot(t, c)] = -av[c];
+        }
+    } else {
+        if (swap) {
+#pragma unroll
+            for (int c = 0; c < kLeaf; ++c) Cs[slot(c, t)] = -(keep(c, t) ? av[c] : T(0));
+        } else {
+#pragma unroll
+            for (int c = 0; c < kLeaf; ++c) Cs[slot(t, c)] = -(keep(t, c) ? av[c] : T(0));
+        }
+    }
 }
 
 template <typename T, bool FWD, bool UNIT>
 __global__ __launch_bounds__(kLeafThreads) void trsm_leaf_kernel(LeafArgs<T> p) {
-    __shared__ __attribute__((aligned(16))) T Cs[kLeaf * kLdsK];  // Cs[k][q][j] = a_{4j+q, k} (solve order)
+    __shared__ __attribute__((aligned(16))) T Cs[kLeaf * kLdsK];  // Cs[k][q][j] = -a_{4j+q, k} (solve order)
     __shared__ T Dg[kLeaf];                      // diagonal in solve order (1 for padding)
     __shared__ T Xs[kLeafVecs * (kLeaf + 1)];    // side L: vectors staged for the transposition
     const int t = threadIdx.x, q = t & (kQuad - 1), vv = t / kQuad;
@@ -182,24 +238,9 @@ __global__ __launch_bounds__(kLeafThreads) void trsm_leaf_kernel(LeafArgs<T> p) 
     const int tc = min(t, nb - 1);
     auto idx = [](int u) { return FWD ? u : kLeaf - 1 - u; };  // solve order -> index in the block
     // Every load of a phase is issued before the first is used (one memory
-    // round trip per phase).  Triangle block, read column by column of the
-    // stored matrix (thread t takes stored row i0 + t, so the wave reads one
-    // contiguous column).  A full block (nb = 64, every leaf but possibly the
-    // last) needs no clamping and no zeroing: every slot a step reads for a
-    // later unknown holds its coefficient, and the other slots only feed
-    // updates that the owner selection discards.
-    const bool full = (nb == kLeaf);
-    const T* ablk = p.a + (int64_t)p.i0 * p.lda + p.i0;
-    T av[kLeaf];
-    if (full) {
-        const T* col = ablk + t;
-#pragma unroll
-        for (int c = 0; c < kLeaf; ++c, col += p.lda) av[c] = *col;
-    } else {
-#pragma unroll
-        for (int c = 0; c < kLeaf; ++c) av[c] = ablk[(int64_t)min(c, nb - 1) * p.lda + tc];
-    }
-    const T dv = ablk[(int64_t)tc * p.lda + tc];  // diagonal of equation t
+    // round trip per phase).
+    T av[kLeaf], dv;
+    leaf_coeffs_load(av, dv, p.a + (int64_t)p.i0 * p.lda + p.i0, p.lda, nb, t);
     T x[kLeafU];
     if (p.left) {
         // vector v is column v of B, unknown i row i0 + i: the wave reads
@@ -216,52 +257,25 @@ __global__ __launch_bounds__(kLeafThreads) void trsm_leaf_kernel(LeafArgs<T> p) 
         }
 #pragma unroll
         for (int w = 0; w < kLeafVecs; ++w) Xs[w * (kLeaf + 1) + t] = bv[w];
-    } else {
-        // vector v is row v of B, unknown i column i0 + i
-        const T* bvec = p.b + (int64_t)p.i0 * p.ldb + min(v, p.nvec - 1);
-#pragma unroll
-        for (int j = 0; j < kLeafU; ++j) x[j] = bvec[(int64_t)min(idx(kQuad * j + q), nb - 1) * p.ldb];
     }
-    if constexpr (!UNIT) Dg[FWD ? t : kLeaf - 1 - t] = (t < nb) ? dv : T(1);
-    // Equation matrix: side L solves op(A) x = b (M = op(A) block), side R
-    // solves x op(A) = b, i.e. op(A)^T x^T = b^T (M = op(A)^T block), so the
-    // stored element (row r, column c) is M[r][c] for side L without
-    // transpose and for side R with it, and M[c][r] otherwise.  Every
-    // (equation, unknown) pair has its own slot.
-    auto slot = [](int i, int k) {  // slot of M[i][k]
-        const int ui = FWD ? i : kLeaf - 1 - i, uk = FWD ? k : kLeaf - 1 - k;
-        return uk * kLdsK + (ui % kQuad) * kLdsQ + ui / kQuad;
-    };
-    auto keep = [&](int i, int k) {  // a coefficient some step reads
-        const int ui = FWD ? i : kLeaf - 1 - i, uk = FWD ? k : kLeaf - 1 - k;
-        return i < nb && k < nb && ui > uk;
-    };
-    const bool swap = (p.left != 0) == (p.trans != 0);
-    if (full) {
-        if (swap) {
+    // vector v is row v of B for side R, unknown i column i0 + i
+    const T* bvec = p.b + (int64_t)p.i0 * p.ldb + min(v, p.nvec - 1);
+    auto load_x = [&](T (&xx)[kLeafU]) {
+        if (p.left) {
 #pragma unroll
-            for (int c = 0; c < kLeaf; ++c) Cs[slot(c, t)] = av[c];
+            for (int j = 0; j < kLeafU; ++j) xx[j] = Xs[vv * (kLeaf + 1) + idx(kQuad * j + q)];
         } else {
 #pragma unroll
-            for (int c = 0; c < kLeaf; ++c) Cs[slot(t, c)] = av[c];
+            for (int j = 0; j < kLeafU; ++j) xx[j] = bvec[(int64_t)min(idx(kQuad * j + q), nb - 1) * p.ldb];
         }
-    } else {
-        if (swap) {
 #pragma unroll
-            for (int c = 0; c < kLeaf; ++c) Cs[slot(c, t)] = keep(c, t) ? av[c] : T(0);
-        } else {
-#pragma unroll
-            for (int c = 0; c < kLeaf; ++c) Cs[slot(t, c)] = keep(t, c) ? av[c] : T(0);
-        }
-    }
+        for (int j = 0; j < kLeafU; ++j)
+            if (idx(kQuad * j + q) >= nb) xx[j] = T(0);
+    };
+    if (!p.left) load_x(x);
+    leaf_coeffs_store<T, FWD, UNIT>(Cs, Dg, av, dv, nb, t, (p.left != 0) == (p.trans != 0));
     __syncthreads();
-    if (p.left) {
-#pragma unroll
-        for (int j = 0; j < kLeafU; ++j) x[j] = Xs[vv * (kLeaf + 1) + idx(kQuad * j + q)];
-    }
-#pragma unroll
-    for (int j = 0; j < kLeafU; ++j)
-        if (idx(kQuad * j + q) >= nb) x[j] = T(0);
+    if (p.left) load_x(x);
     leaf_solve<T, UNIT>(x, Cs + q * kLdsQ, Dg, q, std::make_integer_sequence<int, kLeaf>{});
     if (p.left) {
         __syncthreads();
@@ -284,6 +298,214 @@ __global__ __launch_bounds__(kLeafThreads) void trsm_leaf_kernel(LeafArgs<T> p) 
 #pragma unroll
         for (int j = 0; j < kLeafU; ++j)
             if (idx(kQuad * j + q) < nb) p.b[(int64_t)(p.i0 + idx(kQuad * j + q)) * p.ldb + v] = x[j];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// block solve: up to 256 unknowns per vector in one launch
+// ---------------------------------------------------------------------------
+// Below 256 unknowns the recursion's launches are latency-bound: a 64-row
+// leaf over 8192 vectors and the 64- and 128-deep updates between leaves
+// each take 10-15 us on MI355X for a few microseconds of work.  The block
+// kernel keeps a workgroup's 16 vectors (all 256 unknowns) in LDS and runs
+// the same recursion inside: each 64-block is solved by the leaf's quad
+// solve, and its coupling to every later 64-block of the 256 is subtracted
+// with 16 x 16 x 4 MFMAs, the workgroup's 16 vectors being the MFMA's 16
+// columns (B_t -= M_ts X_s, as the recursion's GEMM updates).
+//
+// Two waves: wave 0 solves, wave 1 updates.  Step k (64-blocks in solve
+// order) has two phases.  A(k): wave 1 subtracts the coupling of block k to
+// block k - 1 (the near update, whose coefficients it read during B(k - 1)),
+// while wave 0 writes block k's coefficients to LDS (read at the end of
+// B(k - 1)).  B(k): wave 0 solves block k; wave 1 reads the
+// coefficients of the near update A(k + 1) and subtracts the coupling of
+// blocks k + 1 and later to block k - 1 (the far updates), which the solve
+// of block k does not need.  Only the solves and the near updates are on
+// the critical path.
+constexpr int kBlock = 256;
+constexpr int kBlockThreads = 2 * kLeafThreads;
+constexpr int kXsV = kBlock + 4;  // LDS row per vector (4 mod 32 doubles: the MFMA reads spread over the banks)
+constexpr int kTileK = kLeaf / 4;  // MFMA k-steps per 64-block
+
+// row of item r of lane l in a 16 x 16 MFMA result (column l % 16)
+template <typename T>
+__device__ __forceinline__ int mfma_d_row(int r, int l);
+template <>
+__device__ __forceinline__ int mfma_d_row<double>(int r, int l) { return 4 * r + (l >> 4); }
+template <>
+__device__ __forceinline__ int mfma_d_row<float>(int r, int l) { return 4 * (l >> 4) + r; }
+
+template <typename T, bool FWD, bool UNIT>
+__global__ __launch_bounds__(kBlockThreads) void trsm_block_kernel(LeafArgs<T> p) {
+    __shared__ __attribute__((aligned(16))) T Cs[kLeaf * kLdsK];  // current 64-block, as in the leaf
+    __shared__ T Dg[kLeaf];
+    __shared__ T Xs[kLeafVecs * kXsV];  // Xs[w][i]: unknown i of vector v0 + w (0 past len)
+    const int tid = threadIdx.x;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int t = tid & (kLeafThreads - 1), q = t & (kQuad - 1), vv = t / kQuad;
+    const int lr = t & 15, lk = t >> 4;  // MFMA operand lane coordinates
+    const int v0 = blockIdx.x * kLeafVecs;
+    const int len = p.nb, nsub = (len + kLeaf - 1) / kLeaf;
+    const bool swap = (p.left != 0) == (p.trans != 0);
+    const gptr<const T> a0 = to_global(p.a) + (int64_t)p.i0 * p.lda + p.i0;
+    const gptr<T> b = to_global(p.b);
+    auto sub = [&](int k) { return FWD ? k : nsub - 1 - k; };  // k-th 64-block in solve order
+    auto idx = [](int u) { return FWD ? u : kLeaf - 1 - u; };
+
+    // wave 0: the 64-blocks' own coefficients
+    T av[kLeaf], dv;
+    auto load_block = [&](int k) {  // coefficients of the k-th 64-block into av, dv
+        const int s0 = sub(k) * kLeaf;
+        int64_t ld = p.lda;
+        asm volatile("" : "+s"(ld));  // the 64 column offsets are recomputed per block, not kept in registers
+        leaf_coeffs_load(av, dv, a0 + (int64_t)s0 * ld + s0, ld, min(kLeaf, len - s0), t);
+    };
+    auto store_block = [&](int k) {
+        leaf_coeffs_store<T, FWD, UNIT>(Cs, Dg, av, dv, min(kLeaf, len - sub(k) * kLeaf), t, swap);
+    };
+
+    // wave 1: coupling tiles.  Tile (kt, rt) of source ks: rows sub(kt) * 64 +
+    // 16 rt + [0, 16) of M against the source block's 64 unknowns; A operand:
+    // lane l holds M[row0 + l % 16][s0 + 4 kk + l / 16].  Indices are clamped
+    // into [0, len): a row past len only feeds a padding unknown (solved
+    // last, coupled to nothing, never stored); a column past len (a partial
+    // first block of a backward solve) is masked to zero before use.
+    const int64_t si = swap ? p.lda : 1, sk = swap ? 1 : p.lda;
+    auto load_tile = [&](int kt, int rt, int ks, T (&am)[kTileK]) {
+        const int s0 = sub(ks) * kLeaf;
+        const gptr<const T> ai = a0 + (int64_t)min(sub(kt) * kLeaf + 16 * rt + lr, len - 1) * si;
+#pragma unroll
+        for (int kk = 0; kk < kTileK; ++kk) am[kk] = ai[(int64_t)min(s0 + 4 * kk + lk, len - 1) * sk];
+        __builtin_amdgcn_sched_barrier(0);  // issued here, consumed later
+    };
+    // B_t -= M_ts X_s as B_t + M_ts (-X_s); xb is -X_s as the B operand:
+    // lane l holds -X[s0 + 4 kk + l / 16][vector l % 16]
+    auto mma_tile = [&](int kt, int rt, int ks, T (&am)[kTileK], const T (&xb)[kTileK]) {
+        const int s0 = sub(ks) * kLeaf;
+        if (len - s0 < kLeaf) {
+#pragma unroll
+            for (int kk = 0; kk < kTileK; ++kk)
+                if (s0 + 4 * kk + lk >= len) am[kk] = T(0);
+        }
+        const int i0 = sub(kt) * kLeaf + 16 * rt;
+        typename Mfma<T>::acc_t acc;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) acc[r] = Xs[lr * kXsV + i0 + mfma_d_row<T>(r, t)];
+#pragma unroll
+        for (int kk = 0; kk < kTileK; ++kk) acc = Mfma<T>::mma(am[kk], xb[kk], acc);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) Xs[lr * kXsV + i0 + mfma_d_row<T>(r, t)] = acc[r];
+    };
+    if (wave == 0) load_block(0);
+    // the vectors, unknowns [i0, i0 + len), into Xs (both waves; every load
+    // is issued before the first store: one memory round trip)
+    {
+        constexpr int kPer = kLeafVecs * kBlock / kBlockThreads;  // elements per thread
+        T xv[kPer];
+        if (p.left) {
+            // vector v0 + w is column v0 + w of B: contiguous rows
+#pragma unroll
+            for (int e = 0; e < kPer; ++e) {
+                const int w = e / (kBlock / kBlockThreads), i = (e % (kBlock / kBlockThreads)) * kBlockThreads + tid;
+                xv[e] = b[(int64_t)min(v0 + w, p.nvec - 1) * p.ldb + p.i0 + min(i, len - 1)];
+            }
+#pragma unroll
+            for (int e = 0; e < kPer; ++e) {
+                const int w = e / (kBlock / kBlockThreads), i = (e % (kBlock / kBlockThreads)) * kBlockThreads + tid;
+                Xs[w * kXsV + i] = (i < len) ? xv[e] : T(0);
+            }
+        } else {
+            // vector v0 + w is row v0 + w of B: lanes w = tid % 16 of one unknown are contiguous
+            const int w = tid & 15;
+            const gptr<const T> row = b + (int64_t)p.i0 * p.ldb + min(v0 + w, p.nvec - 1);
+#pragma unroll
+            for (int e = 0; e < kPer; ++e) xv[e] = row[(int64_t)min(8 * e + (tid >> 4), len - 1) * p.ldb];
+#pragma unroll
+            for (int e = 0; e < kPer; ++e) {
+                const int i = 8 * e + (tid >> 4);
+                Xs[w * kXsV + i] = (i < len) ? xv[e] : T(0);
+            }
+        }
+    }
+    // Each wave runs its own loop (its registers are allocated for its own
+    // work, not for both); the two loops meet at the same barriers, two per
+    // 64-block, which end phases A(k) and B(k).
+    if (wave == 0) {
+        for (int k = 0; k < nsub; ++k) {
+            store_block(k);
+            __syncthreads();  // end of A(k)
+            const int r0 = sub(k) * kLeaf;
+            T x[kLeafU];
+#pragma unroll
+            for (int j = 0; j < kLeafU; ++j) x[j] = Xs[vv * kXsV + r0 + idx(kQuad * j + q)];
+            leaf_solve<T, UNIT>(x, Cs + q * kLdsQ, Dg, q, std::make_integer_sequence<int, kLeaf>{});
+#pragma unroll
+            for (int j = 0; j < kLeafU; ++j) Xs[vv * kXsV + r0 + idx(kQuad * j + q)] = x[j];
+            // the next block's coefficients: in flight during the barrier and
+            // the near update (read during the solve, they cost it issue slots)
+            if (k + 1 < nsub) load_block(k + 1);
+            __syncthreads();  // end of B(k)
+        }
+    } else {
+        T xb[kTileK];     // -X of the last solved block
+        T nm[4][kTileK];  // the near update's coefficients
+        for (int k = 0; k < nsub; ++k) {
+            if (k > 0) {
+#pragma unroll
+                for (int kk = 0; kk < kTileK; ++kk) xb[kk] = -Xs[lr * kXsV + sub(k - 1) * kLeaf + 4 * kk + lk];
+#pragma unroll
+                for (int rt = 0; rt < 4; ++rt) mma_tile(k, rt, k - 1, nm[rt], xb);
+            }
+            __syncthreads();  // end of A(k)
+            if (k + 1 < nsub) {
+#pragma unroll
+                for (int rt = 0; rt < 4; ++rt) load_tile(k + 1, rt, k, nm[rt]);
+            }
+            // far updates: blocks k + 1 .. nsub - 1 from block k - 1, two
+            // operand sets (a tile's loads in flight during the previous tile's MFMAs)
+            const int ntiles = k > 0 ? (nsub - 1 - k) * 4 : 0;
+            if (ntiles > 0) {
+                T am0[kTileK], am1[kTileK];
+                load_tile(k + 1, 0, k - 1, am0);
+                for (int j = 0; j < ntiles; j += 2) {
+                    load_tile(k + 1 + (j + 1) / 4, (j + 1) % 4, k - 1, am1);
+                    mma_tile(k + 1 + j / 4, j % 4, k - 1, am0, xb);
+                    const int jn = min(j + 2, ntiles - 1);
+                    load_tile(k + 1 + jn / 4, jn % 4, k - 1, am0);
+                    mma_tile(k + 1 + (j + 1) / 4, (j + 1) % 4, k - 1, am1, xb);
+                }
+            }
+            __syncthreads();  // end of B(k)
+        }
+    }
+    // X back to B (both waves; every LDS read before the first store)
+    {
+        constexpr int kPer = kLeafVecs * kBlock / kBlockThreads;
+        T xv[kPer];
+        if (p.left) {
+#pragma unroll
+            for (int e = 0; e < kPer; ++e) {
+                const int w = e / (kBlock / kBlockThreads), i = (e % (kBlock / kBlockThreads)) * kBlockThreads + tid;
+                xv[e] = Xs[w * kXsV + i];
+            }
+#pragma unroll
+            for (int e = 0; e < kPer; ++e) {
+                const int w = e / (kBlock / kBlockThreads), i = (e % (kBlock / kBlockThreads)) * kBlockThreads + tid;
+                if (v0 + w < p.nvec && i < len) b[(int64_t)(v0 + w) * p.ldb + p.i0 + i] = xv[e];
+            }
+        } else {
+            const int w = tid & 15;
+#pragma unroll
+            for (int e = 0; e < kPer; ++e) xv[e] = Xs[w * kXsV + 8 * e + (tid >> 4)];
+            if (v0 + w < p.nvec) {
+                const gptr<T> row = b + (int64_t)p.i0 * p.ldb + v0 + w;
+#pragma unroll
+                for (int e = 0; e < kPer; ++e) {
+                    const int i = 8 * e + (tid >> 4);
+                    if (i < len) row[(int64_t)i * p.ldb] = xv[e];
+                }
+            }
+        }
     }
 }
 
@@ -364,7 +586,26 @@ int leaf(const TrsmCtx<T>& c, int i0, int nb) {
     return status_from_hip(hipGetLastError());
 }
 
-// solves the unknowns [i0, i0 + len) (rows of X for side L, columns for side R)
+// Solves the unknowns [i0, i0 + len), 64 < len <= kBlock, in one launch.
+template <typename T>
+int block(const TrsmCtx<T>& c, int i0, int len) {
+    LeafArgs<T> p;
+    p.a = c.a; p.lda = c.lda; p.b = c.b; p.ldb = c.ldb;
+    p.left = c.left; p.lower_eff = c.lower_eff; p.trans = c.trans; p.unit = c.unit;
+    p.i0 = i0; p.nb = len;
+    p.nvec = c.left ? c.n : c.m;
+    const int grid = (p.nvec + kLeafVecs - 1) / kLeafVecs;
+    const bool fwd = c.left ? c.lower_eff : !c.lower_eff;
+    if (fwd) {
+        if (c.unit) trsm_block_kernel<T, true, true><<<grid, kBlockThreads, 0, c.s>>>(p);
+        else trsm_block_kernel<T, true, false><<<grid, kBlockThreads, 0, c.s>>>(p);
+    } else {
+        if (c.unit) trsm_block_kernel<T, false, true><<<grid, kBlockThreads, 0, c.s>>>(p);
+        else trsm_block_kernel<T, false, false><<<grid, kBlockThreads, 0, c.s>>>(p);
+    }
+    return status_from_hip(hipGetLastError());
+}
+
 // A range of unknowns (rows of X for side L, columns for side R).
 struct Range {
     int lo, len;
@@ -417,6 +658,7 @@ constexpr int kTrsmLookaheadMin = 2048;
 template <typename T>
 int solve(const TrsmCtx<T>& c, int i0, int len, bool allow_la = true) {
     if (len <= kLeaf) return leaf(c, i0, len);
+    if (len <= kBlock) return block(c, i0, len);
     Range f, g;
     halves(c, i0, len, f, g);
     int st;
