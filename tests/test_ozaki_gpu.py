@@ -181,6 +181,31 @@ def test_crt_integer_exact_shapes(torch_cuda, shape, ta, tb):
     assert np.array_equal(got, want), mismatch_report(got, want)
 
 
+@pytest.mark.parametrize("ta,tb,beta", [("N", "N", 0.0), ("T", "N", 2.0), ("N", "T", -1.0)])
+def test_crt_parts_integer_exact(torch_cuda, ta, tb, beta):
+    """N of at least 4096: the CRT product runs in column parts on two
+    streams (the second part's operand preparation beside the first part's
+    products, each part's reconstruction beside the next part's products),
+    each part at its own offsets; a ragged last part, transposes, and beta
+    reading C."""
+    m, n, k = 4400, 4300, 150
+    rng = np.random.default_rng(4400 + int(beta))
+    A = rng.integers(-9, 10, (m, k)).astype(np.float64)
+    B = rng.integers(-9, 10, (k, n)).astype(np.float64)
+    C0 = rng.integers(-50, 51, (m, n)).astype(np.float64)
+    a = (A if ta == "N" else A.T).T.ravel()
+    b = (B if tb == "N" else B.T).T.ravel()
+    lda = m if ta == "N" else k
+    ldb = k if tb == "N" else n
+    ldc = m + 3
+    c = np.zeros((n, ldc))
+    c[:, :m] = C0.T
+    got = run(torch_cuda, ta, tb, m, n, k, 1.0, a, lda, b, ldb, beta, c.ravel(), ldc, 3, 14).reshape(n, ldc)
+    want = A @ B + beta * C0  # integers below 2^53: exact
+    assert np.array_equal(got[:, :m].T, want), mismatch_report(got[:, :m].T, want)
+    assert np.array_equal(got[:, m:], c[:, m:])
+
+
 I8_TILES = 26
 BF16_TILES = 4
 

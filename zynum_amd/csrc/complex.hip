@@ -238,7 +238,7 @@ int cgemm_device(char ta, char tb, int m, int n, int k, T alpha_re, T alpha_im, 
     const int64_t ldt = pack_b ? packed_ld(2 * (int64_t)k) / 2 : 0;  // complex elements
     const size_t bt_bytes = pack_b ? (size_t)ldt * n * 2 * sizeof(T) : 0;
 
-    char* ws = static_cast<char*>(workspace_acquire(a2_bytes + bt_bytes + 256));
+    char* ws = static_cast<char*>(workspace_acquire(a2_bytes + bt_bytes + 256, s));
     if (!ws) return ZYNUM_GPU_DECLINE_ALLOC;
     T* a2 = reinterpret_cast<T*>(ws);
     T* bt = reinterpret_cast<T*>(ws + a2_bytes);
@@ -287,9 +287,8 @@ int cgemm_device(char ta, char tb, int m, int n, int k, T alpha_re, T alpha_im, 
             e = gemm_launch<T>(d, s, tile);
         }
     }
-    int st = status_from_hip(e);
-    if (st == ZYNUM_GPU_OK) st = status_from_hip(hipStreamSynchronize(s));  // workspace reuse is host-ordered
-    workspace_release();
+    const int st = status_from_hip(e);
+    workspace_release(s);
     return st;
 }
 

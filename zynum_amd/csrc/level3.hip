@@ -546,10 +546,10 @@ int gemm_update(const TrsmCtx<T>& c, bool ta, bool tb, int m, int n, int k, cons
                 int64_t ldb, T* cc, int64_t ldc) {
     if (m <= 0 || n <= 0 || k <= 0) return ZYNUM_GPU_OK;
     if constexpr (sizeof(T) == 8) {
-        // Split precision pays only on large updates: its operand preparation,
-        // reconstruction and per-call synchronisation are fixed costs that the
-        // small updates at the bottom of the recursion cannot amortise, so
-        // those run on FP64 MFMA (same or smaller error).
+        // Split precision pays only on large updates: its operand preparation
+        // and reconstruction are fixed costs that the small updates at the
+        // bottom of the recursion cannot amortise, so those run on FP64 MFMA
+        // (same or smaller error).
         if (c.mode != ZYNUM_GPU_MODE_PLAIN && std::min(std::min(m, n), k) >= kSplitUpdateMin)
             return ozaki_dgemm(ta, tb, m, n, k, -1.0, a, lda, b, ldb, 1.0, cc, ldc, c.s, c.mode, c.mode_param);
     }
@@ -910,7 +910,7 @@ int symm_device(char side, char uplo, int m, int n, T alpha, const T* a, int64_t
     d.c = c; d.ldc = ldc;
     const int na = left ? m : n;
     const int64_t lds = ld_round(na);
-    T* sym = static_cast<T*>(workspace_acquire(bytes_round((size_t)lds * na * sizeof(T))));
+    T* sym = static_cast<T*>(workspace_acquire(bytes_round((size_t)lds * na * sizeof(T)), s));
     if (!sym) return ZYNUM_GPU_DECLINE_ALLOC;
     const int tiles = (na + 31) / 32;
     const int64_t blocks = (int64_t)tiles * tiles;
@@ -929,9 +929,8 @@ int symm_device(char side, char uplo, int m, int n, T alpha, const T* a, int64_t
         }
         e = gemm_launch_product<T>(d, s);
     }
-    int st = status_from_hip(e);
-    if (st == ZYNUM_GPU_OK) st = status_from_hip(hipStreamSynchronize(s));  // workspace reuse is host-ordered
-    workspace_release();
+    const int st = status_from_hip(e);
+    workspace_release(s);
     return st;
 }
 
@@ -950,7 +949,7 @@ int trmm_device(char side, char uplo, bool trans, char diag, int m, int n, T alp
     const int na = left ? m : n;
     const int64_t ldt = ld_round(na), ldw = ld_round(m);
     const size_t t_bytes = bytes_round((size_t)ldt * na * sizeof(T));
-    char* ws = static_cast<char*>(workspace_acquire(t_bytes + (size_t)ldw * n * sizeof(T)));
+    char* ws = static_cast<char*>(workspace_acquire(t_bytes + (size_t)ldw * n * sizeof(T), s));
     if (!ws) return ZYNUM_GPU_DECLINE_ALLOC;
     T* tri = reinterpret_cast<T*>(ws);
     T* b0 = reinterpret_cast<T*>(ws + t_bytes);
@@ -989,9 +988,8 @@ int trmm_device(char side, char uplo, bool trans, char diag, int m, int n, T alp
         }
         e = gemm_launch<T>(d, s);
     }
-    int st = status_from_hip(e);
-    if (st == ZYNUM_GPU_OK) st = status_from_hip(hipStreamSynchronize(s));
-    workspace_release();
+    const int st = status_from_hip(e);
+    workspace_release(s);
     return st;
 }
 

@@ -1369,7 +1369,7 @@ int ozaki_dgemm(bool ta, bool tb, int m, int n, int k, double alpha, const doubl
     bytes += (plane_a + plane_b) * planes_n + 1024;              // planes
     bytes += crt ? (size_t)ldu * n * planes_n + 256 : (size_t)m * n * 8 + 256;  // u bytes / FP64 level sums
     bytes += 4096;
-    void* wsp = workspace_acquire(bytes);
+    void* wsp = workspace_acquire(bytes, s);
     if (!wsp) return ZYNUM_GPU_DECLINE_ALLOC;
     Workspace ws{static_cast<char*>(wsp), 0};
     uint64_t* max_a = ws.take<uint64_t>(m);
@@ -1501,10 +1501,8 @@ int ozaki_dgemm(bool ta, bool tb, int m, int n, int k, double alpha, const doubl
         }
     }
     int st = status_from_hip(e);
-    const hipError_t es = hipStreamSynchronize(s);  // workspace reuse is host-ordered
-    if (st == ZYNUM_GPU_OK) st = status_from_hip(es);
     if (split && st != ZYNUM_GPU_OK) (void)hipStreamSynchronize(side);
-    workspace_release();
+    workspace_release(s);  // the side stream's work is joined into s
     return st;
 }
 

@@ -26,6 +26,9 @@ std::atomic<int> g_forced{0};
 std::mutex g_ws_mutex;
 void* g_ws_ptr = nullptr;
 size_t g_ws_bytes = 0;
+hipEvent_t g_ws_done = nullptr;    // end of the last user's work
+hipStream_t g_ws_stream = nullptr; // the stream it was recorded on
+bool g_ws_pending = false;         // g_ws_done holds a record
 
 // Staging buffers for host operands: device buffers the kernels read and
 // pinned host bounce buffers the DMA engine copies to and from in one
@@ -504,15 +507,22 @@ void* scratch_acquire(size_t bytes, size_t* got) { return g_stage_dev.acquire(by
 
 void scratch_release(void* p, size_t bytes) { g_stage_dev.release(p, bytes); }
 
-void* workspace_acquire(size_t bytes) {
+void* workspace_acquire(size_t bytes, hipStream_t s) {
     g_ws_mutex.lock();
-    if (bytes <= g_ws_bytes) return g_ws_ptr;
+    if (bytes <= g_ws_bytes) {
+        if (g_ws_pending && g_ws_stream != s && hipStreamWaitEvent(s, g_ws_done, 0) != hipSuccess) {
+            (void)hipGetLastError();
+            (void)hipEventSynchronize(g_ws_done);
+        }
+        return g_ws_ptr;
+    }
     if (g_ws_ptr) {
         (void)hipDeviceSynchronize();
         (void)hipFree(g_ws_ptr);
         g_ws_ptr = nullptr;
         g_ws_bytes = 0;
     }
+    g_ws_pending = false;
     if (hipMalloc(&g_ws_ptr, bytes) != hipSuccess) {
         (void)hipGetLastError();
         g_ws_ptr = nullptr;
@@ -523,7 +533,19 @@ void* workspace_acquire(size_t bytes) {
     return g_ws_ptr;
 }
 
-void workspace_release() { g_ws_mutex.unlock(); }
+void workspace_release(hipStream_t s) {
+    if (g_ws_done == nullptr && hipEventCreateWithFlags(&g_ws_done, hipEventDisableTiming) != hipSuccess)
+        g_ws_done = nullptr;
+    if (g_ws_done != nullptr && hipEventRecord(g_ws_done, s) == hipSuccess) {
+        g_ws_stream = s;
+        g_ws_pending = true;
+    } else {
+        (void)hipGetLastError();
+        (void)hipStreamSynchronize(s);  // no event: the next user finds the work done
+        g_ws_pending = false;
+    }
+    g_ws_mutex.unlock();
+}
 
 void workspace_free_all() {
     g_stage_dev.free_all();
@@ -535,6 +557,7 @@ void workspace_free_all() {
     }
     g_ws_ptr = nullptr;
     g_ws_bytes = 0;
+    g_ws_pending = false;
 }
 
 }  // namespace zg

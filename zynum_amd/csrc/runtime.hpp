@@ -63,8 +63,14 @@ struct Operand {
 // Shared grow-only device workspace (one per process, current device).  The
 // lock is held by the caller for the whole stream-ordered use.
 // workspace_free_all also frees the cached staging buffers.
-void* workspace_acquire(size_t bytes);  // returns nullptr on allocation failure; holds the lock
-void workspace_release();
+// The shared device workspace, stream-ordered: acquire(bytes, s) holds the
+// lock and returns the buffer (nullptr on allocation failure), ready for work
+// queued on s (the previous user's work, if it was queued on another stream,
+// is waited for on the device); release(s) records the end of this user's
+// work on s and drops the lock.  Work of a user that also ran on other
+// streams must be joined into s before release.
+void* workspace_acquire(size_t bytes, hipStream_t s);
+void workspace_release(hipStream_t s);
 void workspace_free_all();
 
 // Device scratch memory from the staging-buffer cache (no lock held, unlike
