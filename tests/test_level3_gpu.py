@@ -57,13 +57,15 @@ def triangle_storage(T, uplo, lda, poison=np.nan):
 COMBOS = [(s, u, t, d) for s in "LR" for u in "LU" for t in "NT" for d in "NU"]
 
 
+@pytest.mark.parametrize("m,n", [(173, 91), (460, 300)])
 @pytest.mark.parametrize("side,uplo,trans,diag", COMBOS)
-def test_dtrsm_integer_exact_recovery(torch_cuda, side, uplo, trans, diag):
+def test_dtrsm_integer_exact_recovery(torch_cuda, side, uplo, trans, diag, m, n):
     """Unit-diagonal integer triangle, integer X: B = op(A) X (or X op(A)) is
     exact and so is every substitution and GEMM update, so X comes back bitwise
-    (m, n not multiples of the 64-row leaf; unstored triangle is NaN)."""
-    rng = np.random.default_rng(sum(ord(ch) * 31**i for i, ch in enumerate(side + uplo + trans + diag)))
-    m, n = 173, 91
+    (m, n not multiples of the 64-row leaf; unstored triangle is NaN).  A
+    triangle of 257-512 unknowns is two block solves with a thin GEMM update
+    between them."""
+    rng = np.random.default_rng(sum(ord(ch) * 31**i for i, ch in enumerate(side + uplo + trans + diag)) + m)
     na = m if side == "L" else n
     A = rng.integers(-1, 2, (na, na)).astype(np.float64)
     T = np.tril(A, -1) if uplo == "L" else np.triu(A, 1)
@@ -87,10 +89,10 @@ def test_dtrsm_integer_exact_recovery(torch_cuda, side, uplo, trans, diag):
     assert np.all(G[:, m:] == -777.0)
 
 
+@pytest.mark.parametrize("m,n", [(200, 150), (460, 300)])
 @pytest.mark.parametrize("side,uplo,trans,diag", COMBOS)
-def test_dtrsm_residual_random(torch_cuda, oracle, side, uplo, trans, diag):
+def test_dtrsm_residual_random(torch_cuda, oracle, side, uplo, trans, diag, m, n):
     rng = np.random.default_rng(7)
-    m, n = 200, 150
     na = m if side == "L" else n
     A = rng.uniform(-1, 1, (na, na))
     A[np.diag_indices(na)] = 8 + np.abs(rng.uniform(-1, 1, na))
