@@ -113,3 +113,36 @@ class Plan:
             self.close()
         except Exception:
             pass
+
+
+def main(argv=None) -> int:
+    """Runs the benchmark batch (4096 problems, M, N, K log-uniform in
+    [16, 512], seed 0) through one plan without graph capture, ``--reps``
+    times, and prints the last run's time: a command for kernel traces and
+    counter collection of the path kernels one launch at a time."""
+    import argparse
+
+    import torch
+
+    ap = argparse.ArgumentParser(prog="python -m zynum_amd.grouped")
+    ap.add_argument("--dtype", choices=("f64", "f32"), default="f64")
+    ap.add_argument("--reps", type=int, default=4)
+    args = ap.parse_args(argv)
+    dt = torch.float64 if args.dtype == "f64" else torch.float32
+    stream = torch.cuda.Stream()
+    batch = Batch(torch, draw_shapes(4096, 0, 16, 512), dt, seed=0)
+    plan = batch.plan()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    for i in range(args.reps):
+        if i == args.reps - 1:
+            e0.record(stream)
+        plan.run(stream.cuda_stream)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    print(f"grouped {args.dtype}: {plan.info()}, last run {e0.elapsed_time(e1):.3f} ms", flush=True)
+    plan.close()
+    return 0
+
+
+if __name__ == "__main__":
+    raise SystemExit(main())
