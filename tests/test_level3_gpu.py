@@ -57,7 +57,7 @@ def triangle_storage(T, uplo, lda, poison=np.nan):
 COMBOS = [(s, u, t, d) for s in "LR" for u in "LU" for t in "NT" for d in "NU"]
 
 
-@pytest.mark.parametrize("m,n", [(173, 91), (460, 300)])
+@pytest.mark.parametrize("m,n", [(173, 91), (460, 300), (200, 1), (1, 250)])
 @pytest.mark.parametrize("side,uplo,trans,diag", COMBOS)
 def test_dtrsm_integer_exact_recovery(torch_cuda, side, uplo, trans, diag, m, n):
     """Unit-diagonal integer triangle, integer X: B = op(A) X (or X op(A)) is
