@@ -98,3 +98,13 @@ def test_scale_only_problems_and_invalid_batch(torch_cuda):
     h = C.c_void_p()
     st = _lib.load().zynum_gpu_grouped_plan_create(_lib.F64, bad.problems, 1, 0, C.byref(h))
     assert st == _lib.DECLINE_ARGUMENT and not h.value
+
+
+def test_benchmark_batch_runner(torch_cuda, capsys):
+    """`python -m zynum_amd.grouped` (the command the profile's counters come
+    from) plans the benchmark batch and runs it without graph capture."""
+    from zynum_amd import grouped
+
+    assert grouped.main(["--dtype", "f32", "--reps", "1"]) == 0
+    out = capsys.readouterr().out
+    assert "grouped f32:" in out and "'tiles': " in out and "last run" in out
