@@ -5,10 +5,11 @@
 // in the reference's blocked template (structured_blocked.zig:315-374):
 // split the triangle in two halves, solve the first, update the second with
 // one GEMM (B2 -= op(A)21 X1, plain FP64 MFMA or split precision), recurse.
-// Leaves of at most 64 rows are solved in FP64 with four lanes per vector
-// (column of B for side L, row for side R) and the vector's unknowns in
-// registers, in the reference's per-step order x_k /= a_kk;
-// x_i = fma(a_ik, -x_k, x_i) (matrix_vector/triangular.zig:375-385).
+// Triangles of at most 256 unknowns are one launch: 64-row blocks solved in
+// FP64 with four lanes per vector (column of B for side L, row for side R)
+// and the vector's unknowns in registers, in the reference's per-step order
+// x_k /= a_kk; x_i = fma(a_ik, -x_k, x_i) (matrix_vector/triangular.zig:
+// 375-385), their coupling subtracted with MFMAs in between.
 //
 // SYRK (symmetric.zig:180-279, one strided dot per element) runs as one launch
 // over the lower (or upper) triangle of output tiles with the GEMM tile routine;
