@@ -397,37 +397,76 @@ __global__ __launch_bounds__(kBlockThreads) void trsm_block_kernel(LeafArgs<T> p
 #pragma unroll
         for (int r = 0; r < 4; ++r) Xs[lr * kXsV + i0 + mfma_d_row<T>(r, t)] = acc[r];
     };
-    if (wave == 0) load_block(0);
-    // the vectors, unknowns [i0, i0 + len), into Xs (both waves; every load
-    // is issued before the first store: one memory round trip)
-    {
-        constexpr int kPer = kLeafVecs * kBlock / kBlockThreads;  // elements per thread
-        T xv[kPer];
+    // The vectors' unknowns of one 64-block, rows [r0, r0 + 64), between B
+    // and Xs, by NTH threads (this one is th): every load (or LDS read) is
+    // issued before the first store.  Rows past len are 0 in Xs and not
+    // written back; vectors past nvec are not written back.
+    auto rows_in = [&](int r0, auto nth_c, int th) {
+        constexpr int NTH = decltype(nth_c)::value, E = kLeaf * kLeafVecs / NTH;
+        T xv[E];
         if (p.left) {
-            // vector v0 + w is column v0 + w of B: contiguous rows
+            // consecutive threads read consecutive rows of one column
 #pragma unroll
-            for (int e = 0; e < kPer; ++e) {
-                const int w = e / (kBlock / kBlockThreads), i = (e % (kBlock / kBlockThreads)) * kBlockThreads + tid;
+            for (int e = 0; e < E; ++e) {
+                const int f = th + NTH * e, w = f / kLeaf, i = r0 + f % kLeaf;
                 xv[e] = b[(int64_t)min(v0 + w, p.nvec - 1) * p.ldb + p.i0 + min(i, len - 1)];
             }
 #pragma unroll
-            for (int e = 0; e < kPer; ++e) {
-                const int w = e / (kBlock / kBlockThreads), i = (e % (kBlock / kBlockThreads)) * kBlockThreads + tid;
+            for (int e = 0; e < E; ++e) {
+                const int f = th + NTH * e, w = f / kLeaf, i = r0 + f % kLeaf;
                 Xs[w * kXsV + i] = (i < len) ? xv[e] : T(0);
             }
         } else {
-            // vector v0 + w is row v0 + w of B: lanes w = tid % 16 of one unknown are contiguous
-            const int w = tid & 15;
-            const gptr<const T> row = b + (int64_t)p.i0 * p.ldb + min(v0 + w, p.nvec - 1);
+            // consecutive threads read consecutive vectors of one row
+            const int w = th % kLeafVecs;
+            const gptr<const T> col = b + (int64_t)p.i0 * p.ldb + min(v0 + w, p.nvec - 1);
 #pragma unroll
-            for (int e = 0; e < kPer; ++e) xv[e] = row[(int64_t)min(8 * e + (tid >> 4), len - 1) * p.ldb];
+            for (int e = 0; e < E; ++e) {
+                const int i = r0 + (th + NTH * e) / kLeafVecs;
+                xv[e] = col[(int64_t)min(i, len - 1) * p.ldb];
+            }
 #pragma unroll
-            for (int e = 0; e < kPer; ++e) {
-                const int i = 8 * e + (tid >> 4);
+            for (int e = 0; e < E; ++e) {
+                const int i = r0 + (th + NTH * e) / kLeafVecs;
                 Xs[w * kXsV + i] = (i < len) ? xv[e] : T(0);
             }
         }
-    }
+    };
+    auto rows_out = [&](int r0, auto nth_c, int th) {
+        constexpr int NTH = decltype(nth_c)::value, E = kLeaf * kLeafVecs / NTH;
+        T xv[E];
+        if (p.left) {
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                const int f = th + NTH * e;
+                xv[e] = Xs[(f / kLeaf) * kXsV + r0 + f % kLeaf];
+            }
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                const int f = th + NTH * e, w = f / kLeaf, i = r0 + f % kLeaf;
+                if (v0 + w < p.nvec && i < len) b[(int64_t)(v0 + w) * p.ldb + p.i0 + i] = xv[e];
+            }
+        } else {
+            const int w = th % kLeafVecs;
+#pragma unroll
+            for (int e = 0; e < E; ++e) xv[e] = Xs[w * kXsV + r0 + (th + NTH * e) / kLeafVecs];
+            if (v0 + w < p.nvec) {
+                const gptr<T> col = b + (int64_t)p.i0 * p.ldb + v0 + w;
+#pragma unroll
+                for (int e = 0; e < E; ++e) {
+                    const int i = r0 + (th + NTH * e) / kLeafVecs;
+                    if (i < len) col[(int64_t)i * p.ldb] = xv[e];
+                }
+            }
+        }
+    };
+    using Both = std::integral_constant<int, kBlockThreads>;
+    using One = std::integral_constant<int, kLeafThreads>;
+
+    if (wave == 0) load_block(0);
+    // the first-solved block's unknowns (both waves); wave 1 reads the others
+    // during its solve, and writes each block back during the next one's
+    rows_in(sub(0) * kLeaf, Both{}, tid);
     // Each wave runs its own loop (its registers are allocated for its own
     // work, not for both); the two loops meet at the same barriers, two per
     // 64-block, which end phases A(k) and B(k).
@@ -462,6 +501,13 @@ __global__ __launch_bounds__(kBlockThreads) void trsm_block_kernel(LeafArgs<T> p
 #pragma unroll
                 for (int rt = 0; rt < 4; ++rt) load_tile(k + 1, rt, k, nm[rt]);
             }
+            if (k == 0) {
+                // the other blocks' unknowns, before the first update reads them
+#pragma unroll 1
+                for (int k2 = 1; k2 < nsub; ++k2) rows_in(sub(k2) * kLeaf, One{}, t);
+            } else {
+                rows_out(sub(k - 1) * kLeaf, One{}, t);  // final since B(k - 1)
+            }
             // far updates: blocks k + 1 .. nsub - 1 from block k - 1, two
             // operand sets (a tile's loads in flight during the previous tile's MFMAs)
             const int ntiles = k > 0 ? (nsub - 1 - k) * 4 : 0;
@@ -479,35 +525,8 @@ __global__ __launch_bounds__(kBlockThreads) void trsm_block_kernel(LeafArgs<T> p
             __syncthreads();  // end of B(k)
         }
     }
-    // X back to B (both waves; every LDS read before the first store)
-    {
-        constexpr int kPer = kLeafVecs * kBlock / kBlockThreads;
-        T xv[kPer];
-        if (p.left) {
-#pragma unroll
-            for (int e = 0; e < kPer; ++e) {
-                const int w = e / (kBlock / kBlockThreads), i = (e % (kBlock / kBlockThreads)) * kBlockThreads + tid;
-                xv[e] = Xs[w * kXsV + i];
-            }
-#pragma unroll
-            for (int e = 0; e < kPer; ++e) {
-                const int w = e / (kBlock / kBlockThreads), i = (e % (kBlock / kBlockThreads)) * kBlockThreads + tid;
-                if (v0 + w < p.nvec && i < len) b[(int64_t)(v0 + w) * p.ldb + p.i0 + i] = xv[e];
-            }
-        } else {
-            const int w = tid & 15;
-#pragma unroll
-            for (int e = 0; e < kPer; ++e) xv[e] = Xs[w * kXsV + 8 * e + (tid >> 4)];
-            if (v0 + w < p.nvec) {
-                const gptr<T> row = b + (int64_t)p.i0 * p.ldb + v0 + w;
-#pragma unroll
-                for (int e = 0; e < kPer; ++e) {
-                    const int i = 8 * e + (tid >> 4);
-                    if (i < len) row[(int64_t)i * p.ldb] = xv[e];
-                }
-            }
-        }
-    }
+    // the last-solved block back to B (both waves)
+    rows_out(sub(nsub - 1) * kLeaf, Both{}, tid);
 }
 
 // ---------------------------------------------------------------------------
