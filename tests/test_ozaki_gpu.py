@@ -206,6 +206,32 @@ def test_crt_parts_integer_exact(torch_cuda, ta, tb, beta):
     assert np.array_equal(got[:, m:], c[:, m:])
 
 
+def test_workspace_is_stream_ordered(torch_cuda):
+    """Two asynchronous split-precision calls on different streams share the
+    library workspace: the second waits on the device for the first (no host
+    synchronisation in between), so both results are exact on integer inputs;
+    then a third call on the first stream after the second."""
+    from zynum_amd import _lib, blas
+
+    torch = torch_cuda
+    m = n = k = 2048
+    g = torch.Generator(device="cuda").manual_seed(11)
+    ops = [(torch.randint(-9, 10, (k, m), generator=g, device="cuda").double(),
+            torch.randint(-9, 10, (n, k), generator=g, device="cuda").double()) for _ in range(3)]
+    outs = [torch.zeros(n * m, dtype=torch.float64, device="cuda") for _ in range(3)]
+    s1, s2 = torch.cuda.Stream(), torch.cuda.Stream()
+    torch.cuda.synchronize()
+    for (a, b), c, s in zip(ops, outs, (s1, s2, s1)):
+        # column-major A (m x k) is a.T's storage: a is (k, m) row-major
+        st = blas.gemm_request(False, False, m, n, k, 1.0, a.data_ptr(), m, b.data_ptr(), k, 0.0, c.data_ptr(), m,
+                               _lib.F64, stream=s.cuda_stream, mode=3, mode_param=14, flags=_lib.FLAG_ASYNC)
+        _lib.check(st, "crt async")
+    torch.cuda.synchronize()
+    for (a, b), c in zip(ops, outs):
+        want = (b @ a).reshape(-1)  # (n, m) row-major = column-major C
+        assert torch.equal(c, want)
+
+
 I8_TILES = 26
 BF16_TILES = 4
 
