@@ -168,7 +168,7 @@ def main() -> int:
             "unit": "TFLOP/s",
             "frac": round(achieved / FP64_MFMA_PEAK_TF, 4),
             "traffic": read_traffic("dgemm_f64_8192_nn_traffic.txt"),
-            "kernel": "gemm_mfma_kernel<double> (one launch per step)",
+            "kernel": "gemm_f64_dma_kernel (one launch per step)",
             "kernel_avg_ms": round(avg_ms, 4),
             "algorithmic_flop_per_launch": flop,
         },

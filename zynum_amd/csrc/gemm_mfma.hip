@@ -17,7 +17,10 @@
 //   * edge tiles (M, N, K not multiples of the tile) use guarded scalar loads
 //     with zero fill; interior tiles run unguarded;
 //   * tile ids are remapped so each of the 8 XCDs gets a contiguous block of
-//     tiles in grouped (super-row) order, keeping shared A/B panels in its L2.
+//     tiles in grouped (super-row) order, keeping shared A/B panels in its L2;
+//   * FP64 NN full tiles of the 16-wave 128x128 configuration skip the
+//     register staging: direct-to-LDS loads fill a 4-stage ring, with the
+//     same MFMA order, so the same results.
 #include <algorithm>
 
 #include "common.hpp"
@@ -40,6 +43,148 @@ void gemm_mfma_kernel(GemmArgs<T> p) {
     int tm, tn;
     grouped_tile(xcd_remap(blockIdx.x, total), p.tiles_m, p.tiles_n, p.group_m, tm, tn);
     gemm_tile<T, CF, TA, TB, VEC>(p, tm * CF::BM, tn * CF::BN, lds_a, lds_b);
+}
+
+// FP64 NN form of the 16-wave 128x128 tile fed by direct-to-LDS loads
+// (global_load_lds_dwordx4), for full tiles: m and n multiples of 128, k of
+// 8, A and B 16-byte aligned with even leading dimensions.  No registers
+// stage the operands: stages of 8 k sit in a 4-stage LDS ring (66 KB, two
+// workgroups per CU as with the register-staged tile), loaded 3 stages
+// ahead, one direct load per wave per stage (waves 0..7 an A k-row of 128
+// rows, waves 8..15 16 columns of B).  Each wave multiplies its 32x32
+// sub-tile in 16x16x4 MFMA steps in increasing k, exactly as gemm_tile does,
+// so results are bitwise those of the register-staged kernel; the freed
+// registers hold the fragments of the next 4-k half, read while the current
+// half's MFMAs run.
+//
+// Stage layout (doubles): A as 8 k-rows of 128 + 2 (the padding of the
+// register-staged tile); B as 128 columns of 8 k, the four 16-byte k pairs of
+// column n rotated by (n >> 2) & 3, so the 16 columns of one fragment read at
+// one k fall on distinct LDS banks.
+constexpr int kDmaStages = 4;
+__global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) void gemm_f64_dma_kernel(
+    GemmArgs<double> p) {
+    constexpr int BM = 128, BN = 128, BK = 8, NS = kDmaStages;
+    constexpr int RA = BM + 2;       // A k-row stride
+    constexpr int SA = BK * RA;      // A part of a stage
+    constexpr int SS = SA + BN * BK;  // stage
+    __shared__ __attribute__((aligned(16))) double lds[NS * SS];
+
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int wm0 = (wave % 4) * 32, wn0 = (wave / 4) * 32;
+    const int total = p.tiles_m * p.tiles_n;
+    int tm, tn;
+    grouped_tile(xcd_remap(blockIdx.x, total), p.tiles_m, p.tiles_n, p.group_m, tm, tn);
+    const int m0 = tm * BM, n0 = tn * BN;
+
+    // this wave's direct load of stage t reads src + t * kstep (2 doubles a
+    // lane) into lds + (t % NS) * SS + dst
+    const double* src;
+    int64_t kstep;
+    int dst;
+    if (wave < 8) {
+        src = p.a + (int64_t)wave * p.lda + m0 + 2 * lane;
+        kstep = (int64_t)BK * p.lda;
+        dst = wave * RA;
+    } else {
+        const int nl = (wave - 8) * 16 + (lane >> 2);  // lane: slot lane % 4 of column nl
+        const int pair = (lane & 3) ^ ((nl >> 2) & 3);
+        src = p.b + (int64_t)(n0 + nl) * p.ldb + 2 * pair;
+        kstep = BK;
+        dst = SA + (wave - 8) * 128;
+    }
+    auto issue = [&](int t) {
+        __builtin_amdgcn_global_load_lds((const void __attribute__((address_space(1)))*)(src + (int64_t)t * kstep),
+                                         (void __attribute__((address_space(3)))*)(lds + (t % NS) * SS + dst), 16, 0,
+                                         0);
+    };
+
+    // fragment offsets of the first 4-k half (k = lk); the second adds 4 k
+    const int lr = lane & 15, lk = lane >> 4;
+    const int ar = Mfma<double>::a_row(lr);
+    int aoff[2], boff[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i) aoff[i] = lk * RA + wm0 + i * 16 + ar;
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+        const int n = wn0 + j * 16 + lr;
+#pragma unroll
+        for (int h = 0; h < 2; ++h) boff[h][j] = SA + n * 8 + ((((4 * h + lk) >> 1) ^ ((n >> 2) & 3)) << 1) + (lk & 1);
+    }
+    v4d acc[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) acc[i][j] = Mfma<double>::zero();
+    double fa[2][2], fb[2][2];  // [half][block]
+    auto read = [&](int h, int t) {
+        const double* base = lds + (t % NS) * SS;
+        fa[h][0] = base[aoff[0] + 4 * h * RA];
+        fa[h][1] = base[aoff[1] + 4 * h * RA];
+        fb[h][0] = base[boff[h][0]];
+        fb[h][1] = base[boff[h][1]];
+    };
+    auto mma_row = [&](int h, int i) {
+#pragma unroll
+        for (int j = 0; j < 2; ++j) acc[i][j] = Mfma<double>::mma(fa[h][i], fb[h][j], acc[i][j]);
+    };
+
+    const int nk = p.k / BK;
+#pragma unroll
+    for (int t = 0; t < NS - 1; ++t)
+        if (t < nk) issue(t);
+    if (nk >= NS - 1) __builtin_amdgcn_s_waitcnt(0x0F72);  // vmcnt(2): this wave's stage 0 has landed
+    else __builtin_amdgcn_s_waitcnt(0x0F70);               // vmcnt(0)
+    __builtin_amdgcn_s_barrier();
+    read(0, 0);
+    for (int s = 0; s < nk; ++s) {
+        // this wave's stage s+1 has landed (stages up to s + NS - 2 are issued)
+        if (s + NS - 2 <= nk - 1) __builtin_amdgcn_s_waitcnt(0x0F71);  // vmcnt(1)
+        else __builtin_amdgcn_s_waitcnt(0x0F70);                       // vmcnt(0)
+        __builtin_amdgcn_sched_barrier(0);
+        __builtin_amdgcn_s_barrier();  // every wave's stage s+1 is in LDS, and every wave has read stage s-1
+        __builtin_amdgcn_sched_barrier(0);
+        if (s + NS - 1 < nk) issue(s + NS - 1);  // into stage s-1's buffer
+        // Each half's fragment reads are issued two MFMAs before the wait
+        // for them (the compiler waits for every outstanding LDS read here).
+        mma_row(0, 0);
+        __builtin_amdgcn_sched_barrier(0);
+        read(1, s);
+        __builtin_amdgcn_sched_barrier(0);
+        mma_row(0, 1);
+        __builtin_amdgcn_sched_barrier(0);
+        mma_row(1, 0);
+        __builtin_amdgcn_sched_barrier(0);
+        read(0, s + 1);  // also on the last step (a valid buffer; the values are unused)
+        __builtin_amdgcn_sched_barrier(0);
+        mma_row(1, 1);
+    }
+
+    const double alpha = p.alpha, beta = p.beta;
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            const int col = n0 + wn0 + j * 16 + lr;
+            const int row = m0 + wm0 + i * 16 + 4 * lk;
+            double* cp = p.c + (int64_t)col * p.ldc + row;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) cp[r] = epilogue<double>(alpha, beta, acc[i][j][r], cp + r);
+        }
+}
+
+// Whether gemm_f64_dma_kernel takes the shape.
+template <typename T>
+static bool dma_eligible(const GemmDesc<T>& d) {
+    if constexpr (sizeof(T) != 8) {
+        return false;
+    } else {
+        const uintptr_t mask = 15;
+        return !d.ta && !d.tb && d.m % 128 == 0 && d.n % 128 == 0 && d.k >= 8 && d.k % 8 == 0 &&
+               (reinterpret_cast<uintptr_t>(d.a) & mask) == 0 && (reinterpret_cast<uintptr_t>(d.b) & mask) == 0 &&
+               d.lda % 2 == 0 && d.ldb % 2 == 0;
+    }
 }
 
 // Split-K form for shapes with few output tiles and a long K: blockIdx.y
@@ -159,7 +304,22 @@ hipError_t gemm_launch_product(const GemmDesc<T>& d, hipStream_t stream, int til
         case kTile256x128w8: return launch_shape<T, 256, 128, 16, 4, 2, 1>(d, stream);
         case kTile128x256w8: return launch_shape<T, 128, 256, 16, 2, 4, 1>(d, stream);
         case kTile128x128w8k8: return launch_shape<T, 128, 128, 8, 4, 2, 2>(d, stream);
-        case kTile128x128w16: return launch_shape<T, 128, 128, 16, 4, 4, 4>(d, stream);
+        case kTile128x128w16:
+            if (dma_eligible(d)) {
+                GemmArgs<double> p;
+                p.m = d.m; p.n = d.n; p.k = d.k;
+                p.lda = d.lda; p.ldb = d.ldb; p.ldc = d.ldc;
+                p.alpha = d.alpha; p.beta = d.beta;
+                p.a = reinterpret_cast<const double*>(d.a);
+                p.b = reinterpret_cast<const double*>(d.b);
+                p.c = reinterpret_cast<double*>(d.c);
+                p.tiles_m = d.m / 128;
+                p.tiles_n = d.n / 128;
+                p.group_m = ZG_GROUP_M;
+                gemm_f64_dma_kernel<<<p.tiles_m * p.tiles_n, 1024, 0, stream>>>(p);
+                return hipGetLastError();
+            }
+            return launch_shape<T, 128, 128, 16, 4, 4, 4>(d, stream);
         default: return hipErrorInvalidValue;
     }
 }
