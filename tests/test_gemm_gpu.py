@@ -267,28 +267,32 @@ def test_every_tile_config_bitwise(torch_cuda, tile, dtype, ta, tb):
 DIRECT_LOAD_TILE = 12  # 1-based request tile of the 16-wave 128x128 configuration
 
 
-@pytest.mark.parametrize("shape,lds", [
-    ((128, 128, 8), (130, 14)),        # one tile, one 8-k stage (direct loads)
-    ((256, 384, 1024), (258, 1030)),   # direct loads, 128 stages
-    ((1024, 640, 2048), (1026, 2050)),  # direct loads, several tiles per XCD
-    ((256, 384, 1020), (258, 1030)),   # k % 8 = 4: register-staged tile
-    ((256, 384, 1024), (257, 1030)),   # odd lda: register-staged tile
-    ((1000, 384, 512), (1002, 514)),   # m not a multiple of 128: register-staged tile
+@pytest.mark.parametrize("ta,tb", [("N", "N"), ("N", "T"), ("T", "N"), ("T", "T")])
+@pytest.mark.parametrize("shape,pads", [
+    ((128, 128, 8), (2, 2)),        # one tile, one 8-k stage (direct loads)
+    ((256, 384, 1024), (2, 6)),     # direct loads, 128 stages
+    ((1024, 640, 2048), (2, 2)),    # direct loads, several tiles per XCD
+    ((256, 384, 1020), (2, 6)),     # k % 8 = 4: register-staged tile
+    ((256, 384, 1024), (1, 6)),     # odd lda: register-staged tile
+    ((1000, 384, 512), (2, 2)),     # m not a multiple of 128: register-staged tile
 ])
-def test_f64_direct_load_tile_integer_exact(torch_cuda, shape, lds):
-    """The FP64 NN 16-wave tile fed by direct-to-LDS loads (full tiles, k a
-    multiple of 8, even leading dimensions) and the shapes just outside it
-    give the exact product on integer inputs with alpha -1/2, beta 2."""
+def test_f64_direct_load_tile_integer_exact(torch_cuda, ta, tb, shape, pads):
+    """The FP64 16-wave tile fed by direct-to-LDS loads (full tiles, k a
+    multiple of 8, even leading dimensions; each transpose pair keeps its own
+    LDS layout) and the shapes just outside it give the exact product on
+    integer inputs with alpha -1/2, beta 2."""
     m, n, k = shape
-    lda, ldb = lds
+    lda = (m if ta == "N" else k) + pads[0]
+    ldb = (k if tb == "N" else n) + pads[1]
     ldc = m + 1
-    a, b, c = operands("N", "N", m, n, k, lda, ldb, ldc, np.float64, True, salts=(3, 8, 1))
-    want = int_gemm("N", "N", m, n, k, -0.5, a, lda, b, ldb, 2.0, c, ldc)
-    got = gpu_gemm(torch_cuda, "N", "N", m, n, k, -0.5, a, lda, b, ldb, 2.0, c, ldc, tile=DIRECT_LOAD_TILE)
+    a, b, c = operands(ta, tb, m, n, k, lda, ldb, ldc, np.float64, True, salts=(3, 8, 1))
+    want = int_gemm(ta, tb, m, n, k, -0.5, a, lda, b, ldb, 2.0, c, ldc)
+    got = gpu_gemm(torch_cuda, ta, tb, m, n, k, -0.5, a, lda, b, ldb, 2.0, c, ldc, tile=DIRECT_LOAD_TILE)
     assert np.array_equal(got, want)
 
 
-def test_f64_direct_load_tile_equals_register_staged(torch_cuda):
+@pytest.mark.parametrize("ta,tb", [("N", "N"), ("N", "T"), ("T", "N"), ("T", "T")])
+def test_f64_direct_load_tile_equals_register_staged(torch_cuda, ta, tb):
     """On random FP64 inputs the direct-load form gives bitwise the result of
     the register-staged tile (same MFMA order): the same matrices stored with
     odd leading dimensions take the register-staged kernel."""
@@ -297,13 +301,15 @@ def test_f64_direct_load_tile_equals_register_staged(torch_cuda):
     A = rs.uniform(-1, 1, (m, k))
     B = rs.uniform(-1, 1, (k, n))
     Cm = rs.uniform(-1, 1, (m, n))
+    opa = A if ta == "N" else A.T.copy()  # the stored matrix (column-major)
+    opb = B if tb == "N" else B.T.copy()
     outs = []
     for pad in (2, 1):  # even leading dimensions: direct loads; odd: register staging
-        lda, ldb, ldc = m + pad, k + pad, m + 3
-        a = np.zeros(lda * k); a.reshape(k, lda)[:, :m] = A.T
-        b = np.zeros(ldb * n); b.reshape(n, ldb)[:, :k] = B.T
+        lda, ldb, ldc = opa.shape[0] + pad, opb.shape[0] + pad, m + 3
+        a = np.zeros(lda * opa.shape[1]); a.reshape(opa.shape[1], lda)[:, : opa.shape[0]] = opa.T
+        b = np.zeros(ldb * opb.shape[1]); b.reshape(opb.shape[1], ldb)[:, : opb.shape[0]] = opb.T
         c = np.zeros(ldc * n); c.reshape(n, ldc)[:, :m] = Cm.T
-        got = gpu_gemm(torch_cuda, "N", "N", m, n, k, 1.25, a, lda, b, ldb, -0.75, c, ldc, tile=DIRECT_LOAD_TILE)
+        got = gpu_gemm(torch_cuda, ta, tb, m, n, k, 1.25, a, lda, b, ldb, -0.75, c, ldc, tile=DIRECT_LOAD_TILE)
         outs.append(logical(got, m, n, ldc).copy())
     assert np.array_equal(outs[0], outs[1])
     ref = 1.25 * (A @ B) - 0.75 * Cm

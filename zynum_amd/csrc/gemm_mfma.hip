@@ -18,9 +18,10 @@
 //     with zero fill; interior tiles run unguarded;
 //   * tile ids are remapped so each of the 8 XCDs gets a contiguous block of
 //     tiles in grouped (super-row) order, keeping shared A/B panels in its L2;
-//   * FP64 NN full tiles of the 16-wave 128x128 configuration skip the
+//   * FP64 full tiles of the 16-wave 128x128 configuration skip the
 //     register staging: direct-to-LDS loads fill a 4-stage ring, with the
-//     same MFMA order, so the same results.
+//     same MFMA order, so the same results; automatic tile choice sends FP64
+//     shapes of at least 512 such tiles there whatever the transpose pair.
 #include <algorithm>
 
 #include "common.hpp"
@@ -45,29 +46,34 @@ void gemm_mfma_kernel(GemmArgs<T> p) {
     gemm_tile<T, CF, TA, TB, VEC>(p, tm * CF::BM, tn * CF::BN, lds_a, lds_b);
 }
 
-// FP64 NN form of the 16-wave 128x128 tile fed by direct-to-LDS loads
+// FP64 form of the 16-wave 128x128 tile fed by direct-to-LDS loads
 // (global_load_lds_dwordx4), for full tiles: m and n multiples of 128, k of
 // 8, A and B 16-byte aligned with even leading dimensions.  No registers
 // stage the operands: stages of 8 k sit in a 4-stage LDS ring (66 KB, two
 // workgroups per CU as with the register-staged tile), loaded 3 stages
-// ahead, one direct load per wave per stage (waves 0..7 an A k-row of 128
-// rows, waves 8..15 16 columns of B).  Each wave multiplies its 32x32
-// sub-tile in 16x16x4 MFMA steps in increasing k, exactly as gemm_tile does,
-// so results are bitwise those of the register-staged kernel; the freed
-// registers hold the fragments of the next 4-k half, read while the current
-// half's MFMAs run.
+// ahead, one direct load per wave per stage (waves 0..7 op(A), waves 8..15
+// op(B)).  Each wave multiplies its 32x32 sub-tile in 16x16x4 MFMA steps in
+// increasing k, exactly as gemm_tile does, so results are bitwise those of
+// the register-staged kernel; the freed registers hold the fragments of the
+// next 4-k half, read while the current half's MFMAs run.
 //
-// Stage layout (doubles): A as 8 k-rows of 128 + 2 (the padding of the
-// register-staged tile); B as 128 columns of 8 k, the four 16-byte k pairs of
-// column n rotated by (n >> 2) & 3, so the 16 columns of one fragment read at
-// one k fall on distinct LDS banks.
+// A direct load cannot transpose, so each operand keeps the layout it has in
+// memory.  An operand contiguous along m / n (A, B^T) is stored as 8 k-rows
+// of 128 + 2 doubles (the padding of the register-staged tile), one direct
+// load per k-row.  An operand contiguous along k (A^T, B) is stored as 128
+// rows of 8 k, 16 rows per direct load, the four 16-byte k pairs of row r
+// rotated by (r >> 2) & 3, so the 16 rows of one fragment read at one k fall
+// on distinct LDS banks.
 constexpr int kDmaStages = 4;
+template <bool TA, bool TB>
 __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) void gemm_f64_dma_kernel(
     GemmArgs<double> p) {
     constexpr int BM = 128, BN = 128, BK = 8, NS = kDmaStages;
-    constexpr int RA = BM + 2;       // A k-row stride
-    constexpr int SA = BK * RA;      // A part of a stage
-    constexpr int SS = SA + BN * BK;  // stage
+    constexpr bool KRA = !TA, KRB = TB;  // operand stored as k-rows (contiguous along m / n)
+    constexpr int RK = BM + 2;            // k-row stride
+    constexpr int SA = KRA ? BK * RK : BM * BK;
+    constexpr int SB = KRB ? BK * RK : BN * BK;
+    constexpr int SS = SA + SB;
     __shared__ __attribute__((aligned(16))) double lds[NS * SS];
 
     const int tid = threadIdx.x, lane = tid & 63;
@@ -78,21 +84,28 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) void 
     grouped_tile(xcd_remap(blockIdx.x, total), p.tiles_m, p.tiles_n, p.group_m, tm, tn);
     const int m0 = tm * BM, n0 = tn * BN;
 
-    // this wave's direct load of stage t reads src + t * kstep (2 doubles a
-    // lane) into lds + (t % NS) * SS + dst
+    // waves 0..7 load op(A), waves 8..15 op(B): a k-row of 128 (k-row layout)
+    // or 16 rows of 8 k (row layout, lane: slot lane % 4 of row lane / 4,
+    // holding k pair (lane % 4) ^ ((row >> 2) & 3))
+    const bool is_a = wave < 8;
+    const int q = is_a ? wave : wave - 8;
+    const bool kr = is_a ? KRA : KRB;
+    const double* x = is_a ? p.a : p.b;
+    const int64_t ldx = is_a ? p.lda : p.ldb;
+    const int x0 = is_a ? m0 : n0;
     const double* src;
     int64_t kstep;
     int dst;
-    if (wave < 8) {
-        src = p.a + (int64_t)wave * p.lda + m0 + 2 * lane;
-        kstep = (int64_t)BK * p.lda;
-        dst = wave * RA;
+    if (kr) {
+        src = x + (int64_t)q * ldx + x0 + 2 * lane;
+        kstep = (int64_t)BK * ldx;
+        dst = (is_a ? 0 : SA) + q * RK;
     } else {
-        const int nl = (wave - 8) * 16 + (lane >> 2);  // lane: slot lane % 4 of column nl
-        const int pair = (lane & 3) ^ ((nl >> 2) & 3);
-        src = p.b + (int64_t)(n0 + nl) * p.ldb + 2 * pair;
+        const int rl = q * 16 + (lane >> 2);
+        const int pair = (lane & 3) ^ ((rl >> 2) & 3);
+        src = x + (int64_t)(x0 + rl) * ldx + 2 * pair;
         kstep = BK;
-        dst = SA + (wave - 8) * 128;
+        dst = (is_a ? 0 : SA) + q * 128;
     }
     auto issue = [&](int t) {
         __builtin_amdgcn_global_load_lds((const void __attribute__((address_space(1)))*)(src + (int64_t)t * kstep),
@@ -100,18 +113,22 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) void 
                                          0);
     };
 
-    // fragment offsets of the first 4-k half (k = lk); the second adds 4 k
     const int lr = lane & 15, lk = lane >> 4;
     const int ar = Mfma<double>::a_row(lr);
-    int aoff[2], boff[2][2];
+    auto at = [](bool krl, int base, int r, int k) {  // element (row r, k) of an operand part
+        return krl ? base + k * RK + r : base + r * BK + ((((k >> 1) ^ ((r >> 2) & 3))) << 1) + (k & 1);
+    };
+    // a k-row operand's second half is a constant 4 k-rows further (folded
+    // into the read's offset); a row-layout operand keeps an offset per half
+    int aoff[KRA ? 1 : 2][2], boff[KRB ? 1 : 2][2];  // [half][block]
 #pragma unroll
-    for (int i = 0; i < 2; ++i) aoff[i] = lk * RA + wm0 + i * 16 + ar;
+    for (int h = 0; h < (KRA ? 1 : 2); ++h)
 #pragma unroll
-    for (int j = 0; j < 2; ++j) {
-        const int n = wn0 + j * 16 + lr;
+        for (int i = 0; i < 2; ++i) aoff[h][i] = at(KRA, 0, wm0 + i * 16 + ar, 4 * h + lk);
 #pragma unroll
-        for (int h = 0; h < 2; ++h) boff[h][j] = SA + n * 8 + ((((4 * h + lk) >> 1) ^ ((n >> 2) & 3)) << 1) + (lk & 1);
-    }
+    for (int h = 0; h < (KRB ? 1 : 2); ++h)
+#pragma unroll
+        for (int i = 0; i < 2; ++i) boff[h][i] = at(KRB, SA, wn0 + i * 16 + lr, 4 * h + lk);
     v4d acc[2][2];
 #pragma unroll
     for (int i = 0; i < 2; ++i)
@@ -120,10 +137,11 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) void 
     double fa[2][2], fb[2][2];  // [half][block]
     auto read = [&](int h, int t) {
         const double* base = lds + (t % NS) * SS;
-        fa[h][0] = base[aoff[0] + 4 * h * RA];
-        fa[h][1] = base[aoff[1] + 4 * h * RA];
-        fb[h][0] = base[boff[h][0]];
-        fb[h][1] = base[boff[h][1]];
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            fa[h][i] = KRA ? base[aoff[0][i] + 4 * h * RK] : base[aoff[KRA ? 0 : h][i]];
+            fb[h][i] = KRB ? base[boff[0][i] + 4 * h * RK] : base[boff[KRB ? 0 : h][i]];
+        }
     };
     auto mma_row = [&](int h, int i) {
 #pragma unroll
@@ -181,7 +199,7 @@ static bool dma_eligible(const GemmDesc<T>& d) {
         return false;
     } else {
         const uintptr_t mask = 15;
-        return !d.ta && !d.tb && d.m % 128 == 0 && d.n % 128 == 0 && d.k >= 8 && d.k % 8 == 0 &&
+        return d.m % 128 == 0 && d.n % 128 == 0 && d.k >= 8 && d.k % 8 == 0 &&
                (reinterpret_cast<uintptr_t>(d.a) & mask) == 0 && (reinterpret_cast<uintptr_t>(d.b) & mask) == 0 &&
                d.lda % 2 == 0 && d.ldb % 2 == 0;
     }
@@ -291,7 +309,13 @@ template <typename T>
 hipError_t gemm_launch_product(const GemmDesc<T>& d, hipStream_t stream, int tile) {
     if (d.m <= 0 || d.n <= 0) return hipSuccess;
     if (d.k <= 0) return scale_launch<T>(d.m, d.n, d.beta, d.c, d.ldc, stream);
-    if (tile == kTileAuto) tile = pick_tile(d.m, d.n, d.k, (int)sizeof(T), d.ta, d.tb);
+    if (tile == kTileAuto) {
+        tile = pick_tile(d.m, d.n, d.k, (int)sizeof(T), d.ta, d.tb);
+        // the direct-load tile beats every register-staged one on large FP64
+        // shapes it takes, for all transpose pairs (TN: 68.2 against 66.5
+        // TFLOP/s for the 8-wave tile the table picks at 8192^3)
+        if (dma_eligible(d) && (int64_t)(d.m / 128) * (d.n / 128) >= 512) tile = kTile128x128w16;
+    }
     switch (tile) {
         case kTile64x64: return launch_shape<T, 64, 64, 16, 2, 2, 0>(d, stream);
         case kTile128x128: return launch_shape<T, 128, 128, 16, 2, 2, 1>(d, stream);
@@ -316,7 +340,11 @@ hipError_t gemm_launch_product(const GemmDesc<T>& d, hipStream_t stream, int til
                 p.tiles_m = d.m / 128;
                 p.tiles_n = d.n / 128;
                 p.group_m = ZG_GROUP_M;
-                gemm_f64_dma_kernel<<<p.tiles_m * p.tiles_n, 1024, 0, stream>>>(p);
+                const int g = p.tiles_m * p.tiles_n;
+                if (!d.ta && !d.tb) gemm_f64_dma_kernel<false, false><<<g, 1024, 0, stream>>>(p);
+                else if (!d.ta && d.tb) gemm_f64_dma_kernel<false, true><<<g, 1024, 0, stream>>>(p);
+                else if (d.ta && !d.tb) gemm_f64_dma_kernel<true, false><<<g, 1024, 0, stream>>>(p);
+                else gemm_f64_dma_kernel<true, true><<<g, 1024, 0, stream>>>(p);
                 return hipGetLastError();
             }
             return launch_shape<T, 128, 128, 16, 4, 4, 4>(d, stream);
