@@ -21,7 +21,7 @@
 //   * FP64 full tiles of the 16-wave 128x128 configuration skip the
 //     register staging: direct-to-LDS loads fill a 4-stage ring, with the
 //     same MFMA order, so the same results; automatic tile choice sends FP64
-//     shapes of at least 512 such tiles there whatever the transpose pair.
+//     shapes of at least 128 such tiles there whatever the transpose pair.
 #include <algorithm>
 
 #include "common.hpp"
@@ -311,10 +311,11 @@ hipError_t gemm_launch_product(const GemmDesc<T>& d, hipStream_t stream, int til
     if (d.k <= 0) return scale_launch<T>(d.m, d.n, d.beta, d.c, d.ldc, stream);
     if (tile == kTileAuto) {
         tile = pick_tile(d.m, d.n, d.k, (int)sizeof(T), d.ta, d.tb);
-        // the direct-load tile beats every register-staged one on large FP64
-        // shapes it takes, for all transpose pairs (TN: 68.2 against 66.5
-        // TFLOP/s for the 8-wave tile the table picks at 8192^3)
-        if (dma_eligible(d) && (int64_t)(d.m / 128) * (d.n / 128) >= 512) tile = kTile128x128w16;
+        // the direct-load tile beats the register-staged ones the table picks
+        // on FP64 shapes it takes from 128 output tiles up, for all transpose
+        // pairs (8192^3 TN: 68.2 against 66.5 TFLOP/s; 2048^3: 54-62 against
+        // 51-58; 1536 x 2048 x 4096 NN / TT: 47.7 / 48.3 against 45.5 / 45.0)
+        if (dma_eligible(d) && (int64_t)(d.m / 128) * (d.n / 128) >= 128) tile = kTile128x128w16;
     }
     switch (tile) {
         case kTile64x64: return launch_shape<T, 64, 64, 16, 2, 2, 0>(d, stream);
