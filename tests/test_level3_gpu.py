@@ -259,6 +259,34 @@ def test_dsyrk_integer_exact_and_triangle_untouched(torch_cuda, uplo, trans, mod
     assert np.all(G[:, n:] == -777.0)
 
 
+@pytest.mark.parametrize("uplo", ["L", "U"])
+@pytest.mark.parametrize("trans", ["N", "T"])
+@pytest.mark.parametrize("n,k,pad", [(384, 264, 2), (384, 264, 1)])
+def test_dsyrk_direct_load_tiles_integer_exact(torch_cuda, uplo, trans, n, k, pad):
+    """Full 128x128 triangle tiles with k a multiple of 8 and an even lda take
+    the direct-to-LDS tile (pad 2), an odd lda the register-staged one (pad 1):
+    both exact on integer inputs, the other triangle and the padding untouched."""
+    rng = np.random.default_rng(11)
+    ar, ac = (n, k) if trans == "N" else (k, n)
+    lda, ldc = ar + pad, n + 2
+    A = rng.integers(-9, 10, (ar, ac)).astype(np.float64)
+    aa = np.full((ac, lda), -555.0)
+    aa[:, :ar] = A.T
+    Cm = rng.integers(-9, 10, (n, n)).astype(np.float64)
+    cc = np.full((n, ldc), -777.0)
+    cc[:, :n] = Cm.T
+    got = syrk(torch_cuda, uplo, trans, n, k, 0.75, aa.ravel(), lda, -0.25, cc.ravel(), ldc)
+    G = got.reshape(n, ldc)
+    X = A if trans == "N" else A.T
+    want = 0.75 * (X.astype(np.int64) @ X.T.astype(np.int64)) + -0.25 * Cm
+    tri = np.tril_indices(n) if uplo == "L" else np.triu_indices(n)
+    other = np.triu_indices(n, 1) if uplo == "L" else np.tril_indices(n, -1)
+    GT = G[:, :n].T
+    assert np.array_equal(GT[tri], want[tri])
+    assert np.array_equal(GT[other], Cm[other])
+    assert np.all(G[:, n:] == -777.0)
+
+
 def test_dsyrk_scale_only(torch_cuda):
     n = 70
     c = np.arange(n * n, dtype=np.float64)

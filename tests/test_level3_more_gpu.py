@@ -173,6 +173,28 @@ def test_syr2k_integer_exact(torch_cuda, oracle, single, uplo, trans, shape):
     assert np.array_equal(G[~stored], C0[~stored])
 
 
+@pytest.mark.parametrize("uplo", "UL")
+@pytest.mark.parametrize("trans", "NT")
+def test_dsyr2k_direct_load_tiles_integer_exact(torch_cuda, uplo, trans):
+    """Full 128x128 triangle tiles, k a multiple of 8 and even leading
+    dimensions: both passes take the direct-to-LDS tile; exact on integers."""
+    n, k = 256, 136
+    rows, cols = (n, k) if trans == "N" else (k, n)
+    lda, ldb, ldc = rows + 2, rows + 4, n + 1
+    a = fill_padded(rows, cols, lda, 5, -999.0, dtype=np.float64, integer=True)
+    b = fill_padded(rows, cols, ldb, 7, -999.0, dtype=np.float64, integer=True)
+    c = fill_padded(n, n, ldc, 9, -777.0, dtype=np.float64, integer=True)
+    A, B = mat(a, rows, cols, lda), mat(b, rows, cols, ldb)
+    if trans == "T":
+        A, B = A.T, B.T
+    full = 2.0 * (A @ B.T + B @ A.T) + 0.5 * mat(c, n, n, ldc)
+    got = run_syr2k(torch_cuda, uplo, trans, n, k, 2.0, a, lda, b, ldb, 0.5, c, ldc, False)
+    G, C0 = mat(got, n, n, ldc), mat(c, n, n, ldc)
+    stored = np.tril(np.ones((n, n), bool)) if uplo == "L" else np.triu(np.ones((n, n), bool))
+    assert np.array_equal(G[stored], full[stored])
+    assert np.array_equal(G[~stored], C0[~stored])
+
+
 @pytest.mark.parametrize("side", "LR")
 def test_symm_random_bound(torch_cuda, oracle, side):
     m, n = 257, 190

@@ -22,6 +22,7 @@
 
 #include "common.hpp"
 #include "gemm.hpp"
+#include "gemm_dma.hpp"
 #include "gemm_tile.hpp"
 #include "ops.hpp"
 #include "runtime.hpp"
@@ -795,9 +796,31 @@ void syrk_kernel(GemmArgs<T> p, int tiles) {
     gemm_tile<T, CF, TA, !TA, VEC, TRI>(p, ti * CF::BM, tj * CF::BN, lds_a, lds_b);
 }
 
-// Triangle-tile launch: 16-byte operand loads when both operands allow them.
+// FP64 triangle tiles on the direct-load tile (full tiles, aligned operands).
+template <bool TA, int TRI>
+__global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) void syrk_dma_kernel(GemmArgs<double> p,
+                                                                                                 int tiles) {
+    int ti, tj;
+    tri_tile<TRI>(xcd_remap(blockIdx.x, tiles), p.tiles_m, ti, tj);
+    gemm_tile_dma<TA, !TA, TRI>(p, ti * 128, tj * 128);
+}
+
+// Triangle-tile launch: the direct-load tile for FP64 full tiles, else the
+// register-staged one with 16-byte operand loads when both operands allow them.
 template <typename T>
 hipError_t syrk_launch(const GemmArgs<T>& p, int tiles, bool lower, bool trans, hipStream_t s) {
+    if constexpr (sizeof(T) == 8) {
+        if (dma_shape_ok(p.m, p.n, p.k, p.a, p.lda, p.b, p.ldb)) {
+            if (lower) {
+                if (trans) syrk_dma_kernel<true, 1><<<tiles, 1024, 0, s>>>(p, tiles);
+                else syrk_dma_kernel<false, 1><<<tiles, 1024, 0, s>>>(p, tiles);
+            } else {
+                if (trans) syrk_dma_kernel<true, 2><<<tiles, 1024, 0, s>>>(p, tiles);
+                else syrk_dma_kernel<false, 2><<<tiles, 1024, 0, s>>>(p, tiles);
+            }
+            return hipGetLastError();
+        }
+    }
     constexpr int V = 16 / sizeof(T);
     const bool vec = ((reinterpret_cast<uintptr_t>(p.a) & 15) == 0) && ((reinterpret_cast<uintptr_t>(p.b) & 15) == 0) &&
                      (p.lda % V == 0) && (p.ldb % V == 0);
