@@ -47,13 +47,13 @@ void gemm_mfma_kernel(GemmArgs<T> p) {
     gemm_tile<T, CF, TA, TB, VEC>(p, tm * CF::BM, tn * CF::BN, lds_a, lds_b);
 }
 
-// FP64 direct-load form of config 11 (gemm_dma.hpp).
-template <bool TA, bool TB>
-__global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) void gemm_f64_dma_kernel(
+// FP64 direct-load form of config 11 (gemm_dma.hpp), NW waves.
+template <bool TA, bool TB, int NW>
+__global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(NW / 2))) void gemm_f64_dma_kernel(
     GemmArgs<double> p) {
     int tm, tn;
     grouped_tile(xcd_remap(blockIdx.x, p.tiles_m * p.tiles_n), p.tiles_m, p.tiles_n, p.group_m, tm, tn);
-    gemm_tile_dma<TA, TB>(p, tm * 128, tn * 128);
+    gemm_tile_dma<TA, TB, NW>(p, tm * 128, tn * 128);
 }
 
 // Whether gemm_f64_dma_kernel takes the shape.
@@ -203,10 +203,11 @@ hipError_t gemm_launch_product(const GemmDesc<T>& d, hipStream_t stream, int til
                 p.tiles_n = d.n / 128;
                 p.group_m = ZG_GROUP_M;
                 const int g = p.tiles_m * p.tiles_n;
-                if (!d.ta && !d.tb) gemm_f64_dma_kernel<false, false><<<g, 1024, 0, stream>>>(p);
-                else if (!d.ta && d.tb) gemm_f64_dma_kernel<false, true><<<g, 1024, 0, stream>>>(p);
-                else if (d.ta && !d.tb) gemm_f64_dma_kernel<true, false><<<g, 1024, 0, stream>>>(p);
-                else gemm_f64_dma_kernel<true, true><<<g, 1024, 0, stream>>>(p);
+                // 4 waves of 32x128 for NN, 8 of 32x64 for the other pairs (fastest of 4 / 8 / 16)
+                if (!d.ta && !d.tb) gemm_f64_dma_kernel<false, false, 4><<<g, 256, 0, stream>>>(p);
+                else if (!d.ta && d.tb) gemm_f64_dma_kernel<false, true, 8><<<g, 512, 0, stream>>>(p);
+                else if (d.ta && !d.tb) gemm_f64_dma_kernel<true, false, 8><<<g, 512, 0, stream>>>(p);
+                else gemm_f64_dma_kernel<true, true, 8><<<g, 512, 0, stream>>>(p);
                 return hipGetLastError();
             }
             return launch_shape<T, 128, 128, 16, 4, 4, 4>(d, stream);

@@ -797,12 +797,13 @@ void syrk_kernel(GemmArgs<T> p, int tiles) {
 }
 
 // FP64 triangle tiles on the direct-load tile (full tiles, aligned operands).
+// 8 waves of 32x64 (the products are NT or TN, where 8 waves are fastest).
 template <bool TA, int TRI>
-__global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) void syrk_dma_kernel(GemmArgs<double> p,
-                                                                                                 int tiles) {
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void syrk_dma_kernel(GemmArgs<double> p,
+                                                                                                int tiles) {
     int ti, tj;
     tri_tile<TRI>(xcd_remap(blockIdx.x, tiles), p.tiles_m, ti, tj);
-    gemm_tile_dma<TA, !TA, TRI>(p, ti * 128, tj * 128);
+    gemm_tile_dma<TA, !TA, 8, TRI>(p, ti * 128, tj * 128);
 }
 
 // Triangle-tile launch: the direct-load tile for FP64 full tiles, else the
@@ -812,11 +813,11 @@ hipError_t syrk_launch(const GemmArgs<T>& p, int tiles, bool lower, bool trans, 
     if constexpr (sizeof(T) == 8) {
         if (dma_shape_ok(p.m, p.n, p.k, p.a, p.lda, p.b, p.ldb)) {
             if (lower) {
-                if (trans) syrk_dma_kernel<true, 1><<<tiles, 1024, 0, s>>>(p, tiles);
-                else syrk_dma_kernel<false, 1><<<tiles, 1024, 0, s>>>(p, tiles);
+                if (trans) syrk_dma_kernel<true, 1><<<tiles, 512, 0, s>>>(p, tiles);
+                else syrk_dma_kernel<false, 1><<<tiles, 512, 0, s>>>(p, tiles);
             } else {
-                if (trans) syrk_dma_kernel<true, 2><<<tiles, 1024, 0, s>>>(p, tiles);
-                else syrk_dma_kernel<false, 2><<<tiles, 1024, 0, s>>>(p, tiles);
+                if (trans) syrk_dma_kernel<true, 2><<<tiles, 512, 0, s>>>(p, tiles);
+                else syrk_dma_kernel<false, 2><<<tiles, 512, 0, s>>>(p, tiles);
             }
             return hipGetLastError();
         }
