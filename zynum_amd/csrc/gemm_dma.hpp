@@ -47,6 +47,10 @@ __device__ __forceinline__ void gemm_tile_dma(const GemmArgs<double>& p, int m0,
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int wm0 = (wave % 4) * 32, wn0 = (wave / 4) * WTN;
 
+    // direct load idx = wave + NW l (0..15) fills part q = idx % 8 of op(A)
+    // (idx < 8) or op(B): k-row q (k-row layout) or rows 16q..16q+15 (row
+    // layout; lane: slot lane % 4 of row lane / 4, holding k pair
+    // (lane % 4) ^ ((row >> 2) & 3)); load t of it reads src + t * kstep
     const double* src[LPW];
     int64_t kstep[LPW];
     int dst[LPW];
