@@ -291,6 +291,34 @@ def test_f64_direct_load_tile_integer_exact(torch_cuda, ta, tb, shape, pads):
     assert np.array_equal(got, want)
 
 
+@pytest.mark.parametrize("ta,tb", [("N", "N"), ("T", "T")])
+def test_f64_direct_load_needs_16_byte_aligned_operands(torch_cuda, ta, tb):
+    """A and B starting one double into their buffers (8-byte aligned) fall
+    back to the register-staged tile, under the automatic choice and with the
+    direct-load configuration forced; both exact on integer inputs."""
+    from zynum_amd import _lib, blas
+
+    torch = torch_cuda
+    m, n, k = 256, 256, 512
+    lda = (m if ta == "N" else k) + 2
+    ldb = (k if tb == "N" else n) + 2
+    ldc = m + 2
+    g = torch.Generator().manual_seed(3)
+    A = torch.randint(-8, 9, (lda * (k if ta == "N" else m) + 1,), generator=g).to(torch.float64).cuda()
+    B = torch.randint(-8, 9, (ldb * (n if tb == "N" else k) + 1,), generator=g).to(torch.float64).cuda()
+    a, b = A[1:].cpu().numpy(), B[1:].cpu().numpy()
+    c = np.zeros(ldc * n)
+    want = int_gemm(ta, tb, m, n, k, 1.0, a, lda, b, ldb, 0.0, c, ldc)
+    assert (A.data_ptr() + 8) % 16 == 8
+    for tile in (0, DIRECT_LOAD_TILE):
+        C_ = torch.zeros(ldc * n, dtype=torch.float64, device="cuda")
+        st = blas.gemm_request(ta == "T", tb == "T", m, n, k, 1.0, A.data_ptr() + 8, lda, B.data_ptr() + 8, ldb, 0.0,
+                               C_.data_ptr(), ldc, _lib.F64, tile=tile)
+        _lib.check(st, "gemm")
+        torch.cuda.synchronize()
+        assert np.array_equal(C_.cpu().numpy(), want)
+
+
 @pytest.mark.parametrize("ta,tb", [("N", "N"), ("N", "T"), ("T", "N"), ("T", "T")])
 def test_f64_direct_load_tile_equals_register_staged(torch_cuda, ta, tb):
     """On random FP64 inputs the direct-load form gives bitwise the result of
